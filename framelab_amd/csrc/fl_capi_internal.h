@@ -1,0 +1,40 @@
+// fl_capi_internal.h -- plan object and launcher declarations shared by the
+// host-side translation units of libframelab_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "fl_geom.h"
+
+struct fl_plan_s {
+    int device = 0;
+    bool cone = false;
+    long long nvox = 0, ndet = 0;
+    int nt = 1, nv = 0;
+    FwdGeom fg{};
+    AdjGeom ag{};
+    std::vector<float> h_sd_phi;
+    float *d_cs = nullptr, *d_sn = nullptr, *d_sdz = nullptr, *d_ydet = nullptr, *d_zdet = nullptr;
+    float *d_cphi = nullptr, *d_sphi = nullptr, *d_cdet = nullptr, *d_sdet = nullptr;
+    int *d_slot_view = nullptr, *d_view_frame = nullptr, *d_idY = nullptr, *d_Nv = nullptr;
+    // lazily allocated workspace (solvers, gradients)
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
+};
+
+int fl_fail(int code, const char* fmt, ...);
+int fl_ok();
+int fl_hip_fail(hipError_t e, const char* what);
+void fl_plan_release_workspace(fl_plan_s* p);
+
+hipError_t fl_launch_fwd_cone(const void* X, int xt, void* Y, int yt, const FwdGeom& g, hipStream_t st);
+hipError_t fl_launch_fwd_fan(const void* X, int xt, void* Y, int yt, const FwdGeom& g, hipStream_t st);
+hipError_t fl_launch_fwd_fan_gao(const void* X, int xt, void* Y, int yt, const FwdGeom& g, hipStream_t st);
+hipError_t fl_launch_trace(const FwdGeom& g, int cone, int nrays, const int* rays, const int* off,
+                           const int* cap, int* J, float* S, int* cnt, hipStream_t st);
+hipError_t fl_launch_ynorm(const float* y, float* yn, const FwdGeom& g, hipStream_t st);
+hipError_t fl_launch_adj_fan(const void* Y, int yt, void* X, int xt, const AdjGeom& g, hipStream_t st);
+hipError_t fl_launch_adj_cone(const void* Y, int yt, void* X, int xt, const AdjGeom& g, hipStream_t st);
+hipError_t fl_launch_adj_matched(const void* Y, int yt, void* X, int xt, const fl_plan_s* p, hipStream_t st);

@@ -1,0 +1,52 @@
+// fl_common.h -- shared device helpers for the FrameLab MI355X kernels.
+//
+// Everything here is compiled with -ffp-contract=off: the projector
+// arithmetic follows the reference's float operation order exactly (each
+// multiply and add rounds), which is what makes voxel indices, lengths and
+// ray sums bit-identical to the reference CPU mex path.  Places that want a
+// fused multiply-add call __builtin_fmaf explicitly.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define FL_INF __builtin_huge_valf()
+
+// Storage element types of volumes and sinograms.  Arithmetic is always
+// FP32; BF16/FP16 are storage formats only (round-to-nearest-even stores).
+enum fl_dtype_t { FL_F32 = 0, FL_BF16 = 1, FL_F16 = 2 };
+
+struct bf16_t { uint16_t v; };
+struct f16_t { _Float16 v; };
+
+__device__ __forceinline__ float fl_load(const float* p, size_t i) { return p[i]; }
+__device__ __forceinline__ float fl_load(const bf16_t* p, size_t i)
+{
+    return __uint_as_float(((uint32_t)p[i].v) << 16);
+}
+__device__ __forceinline__ float fl_load(const f16_t* p, size_t i) { return (float)p[i].v; }
+
+__device__ __forceinline__ uint16_t fl_f32_to_bf16(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ void fl_store(float* p, size_t i, float v) { p[i] = v; }
+__device__ __forceinline__ void fl_store(bf16_t* p, size_t i, float v) { p[i].v = fl_f32_to_bf16(v); }
+__device__ __forceinline__ void fl_store(f16_t* p, size_t i, float v) { p[i].v = (_Float16)v; }
+
+// Map a linear workgroup id so that consecutive logical tiles land on the
+// same XCD (the dispatcher deals workgroups round-robin over 8 XCDs, each
+// with its own L2).  nblk must be the grid size along the remapped axis.
+__device__ __forceinline__ uint32_t fl_xcd_remap(uint32_t b, uint32_t nblk)
+{
+    const uint32_t nx = 8;
+    uint32_t per = nblk / nx, rem = nblk % nx;
+    uint32_t xcd = b % nx, k = b / nx;
+    // XCDs [0, rem) hold per+1 tiles, the rest hold per tiles.
+    uint32_t base = xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per;
+    return base + k;
+}

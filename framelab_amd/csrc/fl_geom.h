@@ -1,0 +1,35 @@
+// fl_geom.h -- device-side geometry bundles passed by value to the kernels.
+#pragma once
+
+#include <stddef.h>
+
+// Forward projector geometry (fan: nz == 0, nb == 1).
+struct FwdGeom {
+    const float* cs;        // cosf(sd_phi[v]) per view (reference forward trig)
+    const float* sn;        // sinf(sd_phi[v])
+    const float* sdz;       // source z per view (cone)
+    const float* ydet;      // detector column coordinates
+    const float* zdet;      // detector row coordinates (cone)
+    const int* slot_view;   // output slot -> view index
+    const int* view_frame;  // view -> image frame (id_X)
+    float SO, OD, scale, dz0;
+    int nx, ny, nz, na, nb, nslots;
+    size_t nvox;            // voxels per frame
+};
+
+// Voxel-driven adjoint geometry (the reference para tables).
+struct AdjGeom {
+    const float* cphi;      // para.cos_phi
+    const float* sphi;      // para.sin_phi
+    const float* cdet;      // para.cos_det (fan)
+    const float* sdet;      // para.sin_det (fan)
+    const float* sdz;       // para.sd_z (cone)
+    const float* ydet;
+    const float* zdet;
+    const int* id_Y;        // frame -> views, tmp_size per frame
+    const int* Nv;          // views per frame
+    float SO, OD, scale, dy_det, y_os, dz_det, dz;
+    int nx, ny, nz, na, nb, nt, tmp_size;
+    size_t nvox;
+    size_t nd;              // detector cells per view
+};

@@ -1,0 +1,220 @@
+"""GPU parity of the HIP projectors against the CPU oracle (and the reference's
+own CPU kernels where oracle/_ref was built).  Bit-exact for the FP32 drop-in
+path: ray sums, traversal (voxel index, length) lists and adjoint gathers."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import bits_equal, first_mismatch, small_cone, small_fan
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _proj(p):
+    from framelab_amd.projector import Projector
+    return Projector(p)
+
+
+def test_fan_forward_bitexact(gpu, oracle):
+    _say("case: fan siddon 64^2, 90 views x 128 bins")
+    p = small_fan()
+    rng = np.random.default_rng(1)
+    X = rng.random((p.ny, p.nx), dtype=np.float32)
+    P = _proj(p)
+    y = P.forward(torch.from_numpy(X).to(gpu)).cpu().numpy().ravel()
+    torch.cuda.synchronize()
+    yo = oracle.siddon_fan_f32(X, p)
+    assert bits_equal(y, yo), first_mismatch(y, yo)
+    if oracle.has_ref():
+        assert bits_equal(y, oracle.ref_siddon_fan(X, p))
+
+
+def test_fan_gao_forward(gpu, oracle):
+    _say("case: fan gao column walk 64^2")
+    p = small_fan()
+    rng = np.random.default_rng(2)
+    X = rng.random((p.ny, p.nx), dtype=np.float32)
+    P = _proj(p)
+    y = P.forward(torch.from_numpy(X).to(gpu), method="gao").cpu().numpy().ravel()
+    ys = oracle.siddon_fan_f32(X, p)
+    # Gao's column walk equals Siddon in exact arithmetic
+    assert np.max(np.abs(y - ys)) <= 1e-5 * np.max(np.abs(ys))
+    if oracle.has_ref():
+        yr = oracle.ref_gao_fan(X, p)
+        assert bits_equal(y, yr), first_mismatch(y, yr)
+
+
+def test_fan_adjoint_bitexact(gpu, oracle):
+    _say("case: fan exact adjoint 64^2")
+    p = small_fan()
+    rng = np.random.default_rng(3)
+    Y = rng.standard_normal((p.nv, p.nd)).astype(np.float32)
+    P = _proj(p)
+    x = P.adjoint(torch.from_numpy(Y).to(gpu)).cpu().numpy().ravel()
+    xo = oracle.atx_fan_f32(Y, p)
+    assert bits_equal(x, xo), first_mismatch(x, xo)
+
+
+@pytest.mark.parametrize("kind", ["circle", "helix", "dz"])
+def test_cone_forward_bitexact(gpu, oracle, kind):
+    _say(f"case: cone siddon 32^3 {kind}")
+    if kind == "circle":
+        p = small_cone()
+    elif kind == "helix":
+        p = small_cone(kind="helix", zmax=0.5, rps=1.0, vtab=1.0, fps=24.0)
+    else:
+        from framelab_amd import geometry as G
+        p = G.cone_para(48, 40, 20, N=(32, 32, 24), L=(2.0, 2.0, 1.2), SO=4.0, OD=4.0,
+                        Ly=G.full_fan_width(), Lz=G.full_cone_height())
+    rng = np.random.default_rng(4)
+    X = rng.random((p.nz, p.ny, p.nx), dtype=np.float32)
+    P = _proj(p)
+    y = P.forward(torch.from_numpy(X).to(gpu)).cpu().numpy().ravel()
+    yo = oracle.siddon_cone_f32(X, p)
+    assert bits_equal(y, yo), first_mismatch(y, yo)
+    if oracle.has_ref():
+        assert bits_equal(y, oracle.ref_siddon_cone(X, p))
+
+
+def test_cone_adjoint_bitexact(gpu, oracle):
+    _say("case: cone exact adjoint 32^3")
+    p = small_cone()
+    rng = np.random.default_rng(5)
+    Y = rng.standard_normal((p.nv, p.nb, p.na)).astype(np.float32)
+    P = _proj(p)
+    x = P.adjoint(torch.from_numpy(Y).to(gpu)).cpu().numpy().ravel()
+    xo = oracle.atx_cone_f32(Y, p)
+    assert bits_equal(x, xo), first_mismatch(x, xo)
+    if oracle.has_ref():
+        assert bits_equal(x, oracle.ref_atx_cone_cpu(Y, p))
+
+
+def test_traversal_bitexact(gpu, oracle):
+    _say("case: traversal logs cone + fan")
+    rng = np.random.default_rng(6)
+    p = small_cone()
+    rays = np.stack([rng.integers(0, p.nv, 300), rng.integers(0, p.na, 300),
+                     rng.integers(0, p.nb, 300)], 1)
+    # axis-aligned views and the central rays
+    extra = [(0, p.na // 2, p.nb // 2), (p.nv // 4, p.na // 2 - 1, p.nb // 2), (p.nv // 2, 0, 0)]
+    rays = np.concatenate([rays, np.array(extra)])
+    P = _proj(p)
+    got = P.trace(rays)
+    want = oracle.siddon_trace(p, rays, cone=True)
+    for k, ((J, S), (Jo, So)) in enumerate(zip(got, want)):
+        assert np.array_equal(J, Jo), f"ray {rays[k]}: voxel sequence differs"
+        assert bits_equal(S, So), f"ray {rays[k]}: lengths differ"
+    pf = small_fan()
+    rays = np.stack([rng.integers(0, pf.nv, 200), rng.integers(0, pf.nd, 200), np.zeros(200, int)], 1)
+    got = _proj(pf).trace(rays)
+    want = oracle.siddon_trace(pf, rays, cone=False)
+    for (J, S), (Jo, So) in zip(got, want):
+        assert np.array_equal(J, Jo) and bits_equal(S, So)
+
+
+def _dot_rel(P, x, y, kind):
+    Ax = P.forward(x).double()
+    Aty = P.adjoint(y, kind=kind).double()
+    lhs = torch.dot(Ax.ravel(), y.double().ravel()).item()
+    rhs = torch.dot(x.double().ravel(), Aty.ravel()).item()
+    return abs(lhs - rhs) / abs(lhs)
+
+
+def test_matched_adjoint_dot(gpu):
+    _say("case: matched adjoint dot tests")
+    from framelab_amd.projector import ADJ_EXACT, ADJ_MATCHED
+    g = torch.Generator(device="cpu").manual_seed(7)
+    for p in (small_cone(), small_fan()):
+        P = _proj(p)
+        x = torch.randn(P.vol_shape, generator=g).to(gpu)
+        y = torch.randn(P.sino_shape, generator=g).to(gpu)
+        rel_m = _dot_rel(P, x, y, ADJ_MATCHED)
+        rel_e = _dot_rel(P, x, y, ADJ_EXACT)
+        _say(f"  dot rel matched={rel_m:.3e} exact={rel_e:.3e}")
+        assert rel_m <= 1e-6
+        assert rel_e <= 1e-3
+
+
+def test_dropin_entry_points(gpu, oracle):
+    _say("case: drop-in host entry points")
+    from framelab_amd import _lib
+    L = _lib.lib()
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+    F = lambda *v: [float(t) for t in v]
+    p = small_cone()
+    rng = np.random.default_rng(8)
+    X = rng.random(p.nx * p.ny * p.nz, dtype=np.float32)
+    y = np.zeros(p.nv * p.na * p.nb, np.float32)
+    L.Ax_cone_mf_gpu_siddon(fp(X), fp(y), fp(p.sd_phi), fp(p.sd_z), fp(p.y_det), fp(p.z_det),
+                            ip(p.id_Y), ip(p.Nv), *F(p.SO, p.OD, p.scale, p.dz), p.nx, p.ny, p.nz, 1,
+                            p.na, p.nb, p.nv, p.tmp_size, 4)
+    _lib.check_last()
+    assert bits_equal(y, oracle.siddon_cone_f32(X, p))
+    yn = np.zeros_like(y)
+    y2 = np.zeros_like(y)
+    L.Ax_cone_mf_gpu_new(fp(X), fp(y2), fp(yn), fp(p.sd_phi), fp(p.sd_z), fp(p.y_det), fp(p.z_det),
+                         ip(p.id_Y), ip(p.Nv), *F(p.SO, p.OD, p.scale, p.dz), p.nx, p.ny, p.nz, 1,
+                         p.na, p.nb, p.nv, p.tmp_size, 4)
+    _lib.check_last()
+    assert bits_equal(y2, y)
+    assert np.all(np.isfinite(yn))
+    Y = rng.standard_normal(y.size).astype(np.float32)
+    x = np.zeros_like(X)
+    L.Atx_cone_mf_gpu_new(fp(x), fp(Y), fp(p.cos_phi), fp(p.sin_phi), fp(p.sd_z), fp(p.y_det),
+                          fp(p.z_det), ip(p.id_Y), ip(p.Nv), *F(p.SO, p.OD, p.scale, p.dy_det, p.y_os,
+                          p.dz_det, p.dz), p.nx, p.ny, p.nz, 1, p.na, p.nb, p.nv, p.tmp_size, 4)
+    _lib.check_last()
+    assert bits_equal(x, oracle.atx_cone_f32(Y, p))
+    pf = small_fan()
+    Xf = rng.random(pf.nx * pf.ny, dtype=np.float32)
+    yf = np.zeros(pf.nv * pf.nd, np.float32)
+    L.Ax_fan_mf_gpu_siddon(fp(Xf), fp(yf), *F(pf.SO, pf.OD, pf.scale), pf.nx, pf.ny, pf.nv, fp(pf.sd_phi),
+                           pf.nd, fp(pf.y_det), ip(pf.id_X), 1)
+    _lib.check_last()
+    assert bits_equal(yf, oracle.siddon_fan_f32(Xf, pf))
+    Yf = rng.standard_normal(yf.size).astype(np.float32)
+    xf = np.zeros_like(Xf)
+    L.Atx_fan_mf_gpu_new(fp(xf), fp(Yf), *F(pf.SO, pf.OD, pf.scale), pf.nx, pf.ny, fp(pf.sd_phi), pf.nd,
+                         fp(pf.y_det), *F(pf.dy_det, pf.y_os), 1, ip(pf.id_Y), ip(pf.Nv), pf.tmp_size,
+                         pf.nv, fp(pf.cos_phi), fp(pf.sin_phi), fp(pf.cos_det), fp(pf.sin_det))
+    _lib.check_last()
+    assert bits_equal(xf, oracle.atx_fan_f32(Yf, pf))
+
+
+def test_invalid_arguments_raise(gpu):
+    _say("case: invalid plan arguments")
+    from framelab_amd import _lib
+    p = small_cone()
+    p.nx = 31  # odd sizes are rejected (planes run -n/2..n/2)
+    with pytest.raises(_lib.FrameLabError):
+        _proj(p)
+    P = _proj(small_cone())
+    with pytest.raises(ValueError):
+        P.forward(torch.zeros(10, device=gpu))
+
+
+def test_lowp_storage(gpu):
+    _say("case: bf16/fp16 storage forward + adjoint")
+    p = small_cone()
+    P = _proj(p)
+    g = torch.Generator(device="cpu").manual_seed(9)
+    x = torch.rand(P.vol_shape, generator=g).to(gpu)
+    y32 = P.forward(x)
+    for dt, tol in ((torch.bfloat16, 1e-2), (torch.float16, 2e-3)):
+        y = P.forward(x.to(dt))
+        assert y.dtype == dt
+        rel = ((y.float() - y32).norm() / y32.norm()).item()
+        assert rel < tol, (dt, rel)
+        xb = P.adjoint(y32.to(dt), dtype=torch.float32)
+        xr = P.adjoint(y32)
+        rel = ((xb - xr).norm() / xr.norm()).item()
+        assert rel < tol, (dt, rel)
