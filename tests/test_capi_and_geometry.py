@@ -1,0 +1,89 @@
+"""CPU-side checks of the C ABI library and the host-side para builders."""
+import ctypes as C
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from framelab_amd import geometry as G
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    names = []
+    for hdr in os.listdir(os.path.join(ROOT, "include")):
+        if not hdr.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", hdr)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"typedef struct[^{]*\{.*?\}[^;]*;", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(", text, flags=re.M):
+            if m.group(1) not in ("if", "while", "for", "defined"):
+                names.append(m.group(1))
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    from framelab_amd import _lib
+    L = _lib.lib()
+    declared = _declared_functions()
+    assert "Ax_cone_mf_gpu_siddon" in declared and "fl_forward" in declared
+    missing = [n for n in declared if not hasattr(L, n)]
+    assert not missing, missing
+    # the ctypes table covers the header
+    assert set(declared) <= set(_lib.SIGNATURES), set(declared) - set(_lib.SIGNATURES)
+
+
+def test_error_strings_without_gpu():
+    from framelab_amd import _lib
+    L = _lib.lib()
+    assert L.fl_error_string(-1) == b"invalid argument"
+    assert L.fl_version() >= 100
+    # invalid geometry is rejected before any device call
+    g = _lib.FlGeometry()
+    g.nx = 31
+    h = C.c_void_p()
+    rc = L.fl_plan_create(C.byref(h), C.byref(g))
+    assert rc == _lib.FL_ERR_INVALID and not h.value
+    assert b"non-positive" in L.fl_last_error_message() or b"even" in L.fl_last_error_message()
+
+
+def test_fan_para_matches_setPara_formulas():
+    """FanBeamScanner/setPara.m:5-30 with the reference defaults SO=OD=5, Ly=6."""
+    p = G.fan_para(256, 180, N=(128, 128), L=(2.0, 2.0))
+    scale = np.float32(2.0) / np.float32(128)
+    assert p.scale == scale
+    assert p.SO == np.float32(5.0) / scale and p.OD == np.float32(5.0) / scale
+    dy = np.float32(6.0) / np.float32(256)
+    assert p.dy_det == np.float32(dy / scale)
+    d = np.diff(p.y_det.astype(np.float64))
+    assert np.allclose(d, float(dy) / float(scale), rtol=1e-6)
+    assert np.allclose(p.y_det + p.y_det[::-1], 0, atol=1e-4)          # symmetric, y_os = 0
+    assert np.allclose(p.sd_phi, 2 * np.pi / 180 * np.arange(180), rtol=1e-6)
+    ang = np.arctan2(p.y_det.astype(np.float64), float(p.SO + p.OD))
+    assert np.allclose(p.cos_det, np.cos(ang), atol=1e-7) and np.allclose(p.sin_det, np.sin(ang), atol=1e-7)
+    assert p.id_X.tolist() == [0] * 180 and p.id_Y.tolist() == list(range(180)) and p.Nv.tolist() == [180]
+
+
+def test_cone_para_matches_setPara_formulas():
+    """ConeBeamScanner/setPara.m:5-56 (circle, and helix z-advance)."""
+    p = G.cone_para(64, 48, 30, N=(32, 32, 16), L=(2.0, 2.0, 1.0))
+    assert p.dz == np.float32(1.0)            # isotropic voxels in voxel units
+    assert np.allclose(np.diff(p.z_det.astype(np.float64)), float(p.dz_det), rtol=1e-5)
+    assert np.all(p.sd_z == 0) and p.nv == 30
+    h = G.cone_para(64, 48, 0, N=(32, 32, 32), kind="helix", zmax=2.0, rps=3.0, vtab=2.0, fps=30.0)
+    assert h.nv == int(math.floor(30 * 2.0 / 2.0)) + 1          # ConeBeamScanner.m:55
+    # z advances vtab/(2*pi*rps) per radian, in voxel units
+    slope = np.polyfit(h.sd_phi.astype(np.float64), h.sd_z.astype(np.float64), 1)[0]
+    assert abs(slope - 2.0 / (2 * np.pi * 3.0) / float(h.scale)) < 1e-3 * abs(slope)
+    with pytest.raises(ValueError):
+        G.cone_para(8, 8, 8, kind="doubleHelix")
+
+
+def test_benchmark_detector_extents():
+    """Full fan covers the circumscribed circle r=sqrt(2): half-angle asin(sqrt2/4)."""
+    assert abs(G.full_fan_width() - 6.047432) < 1e-5
+    assert abs(G.full_cone_height() - 6.187673) < 1e-5

@@ -1,0 +1,254 @@
+"""Pin the CPU oracle (oracle/ct_oracle.c) before trusting it.
+
+Each restated function is checked against (a) the committed golden vectors
+written by the reference's own CPU kernels (tests/golden), (b) those kernels
+directly when oracle/_ref was built here, and (c) closed forms and brute
+force in FP64 chosen so that a dropped term, a wrong sign or index, or a
+transposed operand fails.
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import bits_equal, first_mismatch, small_cone, small_fan
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+import cases  # noqa: E402
+
+GOLD = np.load(os.path.join(HERE, "golden", "golden_projectors.npz"))
+
+
+# ---------------------------------------------------------------- golden ---
+
+def test_golden_merge(oracle):
+    for k, (a, b) in enumerate(cases.merge_cases()):
+        assert bits_equal(oracle.merge_union(a, b), GOLD[f"merge_{k}"]), f"case {k}"
+
+
+def test_golden_fan(oracle):
+    for k, (p, X) in enumerate(cases.fan_cases()):
+        y = oracle.siddon_fan_f32(X, p)
+        assert bits_equal(y, GOLD[f"fan_siddon_{k}"]), first_mismatch(y, GOLD[f"fan_siddon_{k}"])
+
+
+def test_golden_cone(oracle):
+    for k, (p, X, Y, adj_ok) in enumerate(cases.cone_cases()):
+        y = oracle.siddon_cone_f32(X, p)
+        assert bits_equal(y, GOLD[f"cone_siddon_{k}"]), first_mismatch(y, GOLD[f"cone_siddon_{k}"])
+        if adj_ok:
+            x = oracle.atx_cone_f32(Y, p)
+            assert bits_equal(x, GOLD[f"cone_adj_{k}"]), first_mismatch(x, GOLD[f"cone_adj_{k}"])
+
+
+def test_golden_find_l_3d(oracle):
+    args = cases.find_l_3d_args()
+    got = np.array([oracle.find_l_3d(*map(float, a)) for a in args], np.float32)
+    assert bits_equal(got, GOLD["find_l_3d"]), first_mismatch(got, GOLD["find_l_3d"])
+    # known answers of matlab/mex/src2/test_find_l_3d.cpp: a z-directed ray
+    # misses the box at (5,5,5) and runs its full dz=0.25 through (0,0,0.5)
+    assert got[0] == 0.0 and got[1] == np.float32(0.25)
+
+
+# ------------------------------------------------ the reference, directly --
+
+needs_ref = pytest.mark.skipif(not os.path.exists(os.path.join(HERE, "..", "oracle", "_ref",
+                                                               "libframelab_ref.so")),
+                               reason="oracle/_ref not built (no reference checkout)")
+
+
+@needs_ref
+def test_merge_vs_reference_random(oracle):
+    rng = np.random.default_rng(11)
+    for _ in range(300):
+        a = np.unique(rng.integers(0, 60, rng.integers(1, 30)).astype(np.float32) / 7)
+        b = np.unique(rng.integers(0, 60, rng.integers(1, 30)).astype(np.float32) / 7)
+        assert bits_equal(oracle.merge_union(a, b), oracle.ref_sort_alpha(a, b))
+
+
+@needs_ref
+@pytest.mark.parametrize("seed", range(4))
+def test_fan_vs_reference(oracle, seed):
+    rng = np.random.default_rng(seed)
+    N = int(rng.choice([8, 16, 30, 64]))
+    p = small_fan(nd=int(rng.integers(8, 100)) * 2, nv=int(rng.integers(3, 40)), N=N)
+    X = rng.standard_normal((N, N)).astype(np.float32)
+    assert bits_equal(oracle.siddon_fan_f32(X, p), oracle.ref_siddon_fan(X, p))
+
+
+@needs_ref
+@pytest.mark.parametrize("seed", range(3))
+def test_cone_vs_reference(oracle, seed):
+    rng = np.random.default_rng(seed + 10)
+    p = small_cone(na=int(rng.integers(4, 40)) * 2, nb=int(rng.integers(4, 30)) * 2,
+                   nv=int(rng.integers(2, 12)), N=int(rng.choice([8, 16, 24])))
+    X = rng.standard_normal((p.nz, p.ny, p.nx)).astype(np.float32)
+    assert bits_equal(oracle.siddon_cone_f32(X, p), oracle.ref_siddon_cone(X, p))
+    Y = rng.standard_normal((p.nv, p.nb, p.na)).astype(np.float32)
+    assert bits_equal(oracle.atx_cone_f32(Y, p), oracle.ref_atx_cone_cpu(Y, p))
+
+
+@needs_ref
+def test_chords_vs_reference(oracle):
+    rng = np.random.default_rng(12)
+    R = oracle.ref()
+    for _ in range(2000):
+        a = rng.uniform(-4, 4, 12).astype(np.float32)
+        a[6:9] = np.abs(a[6:9]) + 0.25
+        assert oracle.find_l_3d(*map(float, a)) == R.ref_find_l_3d(*map(float, a))
+        b = rng.uniform(-4, 4, 8).astype(np.float32)
+        b[4:6] = np.abs(b[4:6]) + 0.25
+        assert oracle.find_l2(*map(float, b)) == R.ref_find_l2(*map(float, b))
+
+
+# --------------------------------------------- closed forms / brute force --
+
+def _box_chord_np(p0, p1, c, h):
+    """FP64 slab clipping in numpy (independent of the C chord)."""
+    d = p1 - p0
+    t0, t1 = -np.inf, np.inf
+    for k in range(len(p0)):
+        if d[k] == 0:
+            if abs(p0[k] - c[k]) > h[k]:
+                return 0.0
+            continue
+        a, b = (c[k] - h[k] - p0[k]) / d[k], (c[k] + h[k] - p0[k]) / d[k]
+        t0, t1 = max(t0, min(a, b)), min(t1, max(a, b))
+    return max(0.0, t1 - t0) * float(np.linalg.norm(d))
+
+
+def test_chord_functions_closed_form(oracle):
+    rng = np.random.default_rng(13)
+    for _ in range(1500):
+        s = rng.uniform(-6, 6, 3)
+        e = rng.uniform(-6, 6, 3)
+        c = rng.uniform(-1, 1, 3)
+        h = np.array([0.5, 0.5, 0.5])
+        want = _box_chord_np(s, e, c, h)
+        assert abs(oracle.chord_box_f64(s, e, c, h) - want) < 1e-12
+        got = oracle.find_l_3d(*s, *e, 1.0, 1.0, 1.0, *c)
+        assert abs(got - want) <= 2e-5 * (1 + abs(want)), (s, e, c, got, want)
+        got2 = oracle.find_l2(s[0], s[1], e[0], e[1], 1.0, 1.0, c[0], c[1])
+        want2 = _box_chord_np(s[:2], e[:2], c[:2], h[:2])
+        assert abs(got2 - want2) <= 2e-5 * (1 + want2)
+        # line form a*x+b*y=c with unit normal (find_l.cpp): same chord
+        d = e[:2] - s[:2]
+        nrm = np.array([-d[1], d[0]]) / np.linalg.norm(d)
+        got3 = oracle.find_l(nrm[0], nrm[1], float(nrm @ s[:2]), c[0], c[1])
+        assert abs(got3 - want2) <= 1e-4 * (1 + want2)
+
+
+def test_siddon_f64_brute_force(oracle):
+    """FP64 Siddon ray sums equal sum_v chord(v)*x_v over all voxels."""
+    rng = np.random.default_rng(14)
+    p = small_cone(na=6, nb=5, nv=3, N=4)
+    X = rng.random((4, 4, 4))
+    y = oracle.siddon_f64(X, p).reshape(p.nv, p.nb, p.na)
+    centres = [(ix + 0.5 - 2, iy + 0.5 - 2, (iz + 0.5 - 2) * float(p.dz))
+               for iz in range(4) for iy in range(4) for ix in range(4)]
+    h = np.array([0.5, 0.5, 0.5 * float(p.dz)])
+    for iv in range(p.nv):
+        c, s = math.cos(float(p.sd_phi[iv])), math.sin(float(p.sd_phi[iv]))
+        src = np.array([-float(p.SO) * c, -float(p.SO) * s, 0.0])
+        for ib in range(p.nb):
+            for ia in range(p.na):
+                yd = float(p.y_det[ia])
+                det = np.array([c * float(p.OD) - s * yd, s * float(p.OD) + c * yd, float(p.z_det[ib])])
+                want = sum(_box_chord_np(src, det, np.array(cc), h) * X.ravel()[k]
+                           for k, cc in enumerate(centres)) * float(p.scale)
+                assert abs(y[iv, ib, ia] - want) < 1e-12 * (1 + abs(want))
+
+
+def test_siddon_f32_ones_volume_is_box_chord(oracle):
+    """A of an all-ones volume is the chord of the volume's bounding box."""
+    p = small_cone(na=40, nb=30, nv=7, N=16)
+    y = oracle.siddon_cone_f32(np.ones((16, 16, 16), np.float32), p).reshape(p.nv, p.nb, p.na)
+    c_, s_ = oracle.trig_f32(p.sd_phi)
+    h = np.array([8.0, 8.0, 8.0 * float(p.dz)])
+    for iv in range(p.nv):
+        c, s = float(c_[iv]), float(s_[iv])
+        src = np.array([-float(p.SO) * c, -float(p.SO) * s, 0.0])
+        for ib in range(0, p.nb, 3):
+            for ia in range(0, p.na, 3):
+                yd = float(p.y_det[ia])
+                det = np.array([c * float(p.OD) - s * yd, s * float(p.OD) + c * yd, float(p.z_det[ib])])
+                want = _box_chord_np(src, det, np.zeros(3), h) * float(p.scale)
+                # FP32 alpha differences: 1e-5 relative plus an absolute floor of
+                # a few ulps of the source-detector parameter times |d|*scale
+                assert abs(y[iv, ib, ia] - want) <= 1e-5 * want + 4e-6, (iv, ib, ia)
+
+
+def test_disc_chord_kat(oracle):
+    """test/testBall.m:3-21: a unit disc with SO=OD=5 projects to
+    l(a) = 4*sqrt(-6 + 625/(100+a^2)); discretised at 512^2 within 1e-2."""
+    from framelab_amd import geometry as G
+    n = 512
+    p = G.fan_para(64, 4, N=(n, n), L=(2.2, 2.2), SO=5.0, OD=5.0, Ly=6.0)
+    c = -1.1 + (np.arange(n) + 0.5) * 2.2 / n
+    Yg, Xg = np.meshgrid(c, c, indexing="ij")
+    X = (Xg ** 2 + Yg ** 2 <= 1.0).astype(np.float64)
+    y = oracle.siddon_f64(X, p, cone=False).reshape(p.nv, p.nd)
+    a = p.y_det.astype(np.float64) * float(p.scale)
+    want = 4 * np.sqrt(np.maximum(-6 + 625 / (100 + a ** 2), 0))
+    for iv in range(p.nv):
+        assert np.max(np.abs(y[iv] - want)) < 1e-2
+
+
+def test_fan_adjoint_column_sums(oracle):
+    """A^T 1 of the exact fan adjoint equals the brute-force sum of FP64
+    pixel chords over the rays (pins the window and the chord routine)."""
+    p = small_fan(nd=24, nv=5, N=8)
+    x = oracle.atx_fan_f32(np.ones(p.nv * p.nd, np.float32), p).reshape(8, 8)
+    want = np.zeros((8, 8))
+    for iv in range(p.nv):
+        c, s = float(p.cos_phi[iv]), float(p.sin_phi[iv])
+        src = np.array([-float(p.SO) * c, -float(p.SO) * s])
+        for ia in range(p.nd):
+            yd = float(p.y_det[ia])
+            det = np.array([c * float(p.OD) - s * yd, s * float(p.OD) + c * yd])
+            for iy in range(8):
+                for ix in range(8):
+                    want[iy, ix] += _box_chord_np(src, det, np.array([ix - 3.5, iy - 3.5]),
+                                                  np.array([0.5, 0.5]))
+    want *= float(p.scale)
+    assert np.max(np.abs(x - want)) <= 1e-4 * np.max(want)
+
+
+def test_dot_products(oracle):
+    """<Ax,y> = <x,A^T y>: FP64 pair to 1e-13; FP32 Siddon vs exact chord
+    adjoints to the float-chord level with positive data (the reference's
+    testAdjoint.m:10-23 uses rand)."""
+    rng = np.random.default_rng(15)
+    p = small_cone(na=40, nb=32, nv=10, N=16)
+    X = rng.random((16, 16, 16))
+    Y = rng.random(p.nv * p.nb * p.na)
+    lhs = oracle.siddon_f64(X, p) @ Y
+    rhs = X.ravel() @ oracle.siddon_adj_f64(Y, p)
+    assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
+    X32, Y32 = X.astype(np.float32), Y.astype(np.float32)
+    lhs = oracle.siddon_cone_f32(X32, p).astype(np.float64) @ Y32.astype(np.float64)
+    rhs = X32.ravel().astype(np.float64) @ oracle.atx_cone_f32(Y32, p).astype(np.float64)
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+    pf = small_fan(nd=64, nv=20, N=32)
+    Xf = rng.random((32, 32)).astype(np.float32)
+    Yf = rng.random(pf.nv * pf.nd).astype(np.float32)
+    lhs = oracle.siddon_fan_f32(Xf, pf).astype(np.float64) @ Yf.astype(np.float64)
+    rhs = Xf.ravel().astype(np.float64) @ oracle.atx_fan_f32(Yf, pf).astype(np.float64)
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+
+
+def test_trace_matches_projection(oracle):
+    """The traversal log reproduces the ray sums of the FP32 forward."""
+    rng = np.random.default_rng(16)
+    p = small_cone(na=16, nb=12, nv=4, N=8)
+    X = rng.random((8, 8, 8)).astype(np.float32)
+    y = oracle.siddon_cone_f32(X, p).reshape(p.nv, p.nb, p.na)
+    rays = [(iv, ia, ib) for iv in range(p.nv) for ib in range(p.nb) for ia in range(p.na)]
+    for (iv, ia, ib), (J, S) in zip(rays, oracle.siddon_trace(p, rays)):
+        acc = np.float32(0)
+        for j, s in zip(J, S):
+            acc = np.float32(acc + np.float32(s * X.ravel()[j]))
+        assert np.float32(acc * p.scale) == y[iv, ib, ia]
