@@ -105,6 +105,172 @@ __global__ __launch_bounds__(256) void k_adj_cone(const YT* __restrict__ Y, XT* 
     fl_store(X, (size_t)it * g.nvox + ((size_t)iz * g.ny + iy) * g.nx + ix, acc * g.scale);
 }
 
+// Cone, footprint-staged: per view the workgroup stages its detector
+// footprint in LDS -- the rays' direction components, the six direction
+// ratios find_l_3d divides out (each computed once per workgroup by the same
+// IEEE division instead of once per voxel), and the sinogram values.  Each
+// voxel then skips (ray, voxel) pairs that provably miss (the line passes
+// more than 0.02 voxel outside the box, where the float face tests of the
+// chord routine cannot succeed) and evaluates the chord of the rest with the
+// reference arithmetic, summing in the reference order.  Skipped pairs are
+// exact zeros, so the result is bit-identical to k_adj_cone.
+namespace {
+constexpr int FP_A = 48;    // footprint columns held in LDS
+constexpr int FP_B = 24;    // footprint rows
+constexpr float MISS_MARGIN = 0.02f;
+
+__device__ __forceinline__ int wave_min(int v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+}  // namespace
+
+template <typename YT, typename XT>
+__global__ __launch_bounds__(256) void k_adj_cone_fp(const YT* __restrict__ Y, XT* __restrict__ X, AdjGeom g)
+{
+    __shared__ float c_a[FP_A], c_b[FP_A], c_sba[FP_A], c_sab[FP_A];
+    __shared__ float c_nx[FP_A], c_ny[FP_A], c_off[FP_A], c_hs[FP_A];
+    __shared__ float c_ta[FP_A], c_tb[FP_A], c_t0[FP_A], c_hw[FP_A];
+    __shared__ float r_c[FP_B];
+    __shared__ float4 q_r[FP_A * FP_B];
+    __shared__ float q_y[FP_A * FP_B];
+    __shared__ int red[2][4][4];
+
+    const int bx = (g.nx + 7) / 8, by = (g.ny + 7) / 8, bz = (g.nz + 3) / 4;
+    const uint32_t nblk = (uint32_t)bx * by * bz;
+    const uint32_t b = fl_xcd_remap(blockIdx.x, nblk);
+    const int it = blockIdx.y;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int ix = (int)(b % bx) * 8 + (tid & 7);
+    const int iy = (int)((b / bx) % by) * 8 + ((tid >> 3) & 7);
+    const int iz = (int)(b / ((uint32_t)bx * by)) * 4 + (tid >> 6);
+    const bool active = ix < g.nx && iy < g.ny && iz < g.nz;
+
+    const int na = g.na, nb = g.nb, na2 = na / 2, nb2 = nb / 2;
+    const float SD = g.SO + g.OD;
+    const float d = sqrtf((1 + g.dz * g.dz) / 2);
+    const float zc = ((float)iz + 0.5f - (float)(g.nz / 2)) * g.dz;
+    const float yc = (float)iy + 0.5f - (float)(g.ny / 2);
+    const float xc = (float)ix + 0.5f - (float)(g.nx / 2);
+    const float hz = g.dz / 2;
+    float acc = 0.0f;
+    const int nvf = g.Nv[it];
+    for (int j = 0; j < nvf; j++) {
+        const int iv = g.id_Y[it * g.tmp_size + j];
+        const YT* __restrict__ y = Y + (size_t)iv * g.nd;
+        const float c = g.cphi[iv], s = g.sphi[iv];
+        const float sz = g.sdz[iv];
+        int alo = na, ahi = -1, blo = nb, bhi = -1;
+        if (active) {
+            const float xr = c * xc + s * yc;
+            const float yr = -s * xc + c * yc;
+            const float zr = zc - sz;
+            float t = SD / ((xr + g.SO) * g.dy_det);
+            ahi = (int)ceilf((yr + 1) * t - g.y_os + (float)na2);
+            alo = (int)floorf((yr - 1) * t - g.y_os + (float)na2);
+            t = SD / ((xr + g.SO) * g.dz_det);
+            bhi = (int)ceilf((zr + d) * t + (float)nb2);
+            blo = (int)floorf((zr - d) * t + (float)nb2);
+            if (alo < 0) alo = 0;
+            if (ahi > na - 1) ahi = na - 1;
+            if (blo < 0) blo = 0;
+            if (bhi > nb - 1) bhi = nb - 1;
+            if (alo > ahi || blo > bhi) { alo = na; ahi = -1; blo = nb; bhi = -1; }
+        }
+        // workgroup footprint = union of the voxels' windows
+        {
+            int m0 = wave_min(alo), m1 = wave_max(ahi), m2 = wave_min(blo), m3 = wave_max(bhi);
+            if (lane == 0) {
+                red[j & 1][wv][0] = m0; red[j & 1][wv][1] = m1;
+                red[j & 1][wv][2] = m2; red[j & 1][wv][3] = m3;
+            }
+        }
+        __syncthreads();
+        int A0 = red[j & 1][0][0], A1 = red[j & 1][0][1], B0 = red[j & 1][0][2], B1 = red[j & 1][0][3];
+#pragma unroll
+        for (int w = 1; w < 4; w++) {
+            A0 = min(A0, red[j & 1][w][0]); A1 = max(A1, red[j & 1][w][1]);
+            B0 = min(B0, red[j & 1][w][2]); B1 = max(B1, red[j & 1][w][3]);
+        }
+        const int nA = A1 - A0 + 1, nB = B1 - B0 + 1;
+        if (nA <= 0 || nB <= 0) continue;   // uniform: no voxel of the brick sees this view
+        const float x1 = c * (-g.SO), y1 = s * (-g.SO), z1 = sz;
+        if (nA <= FP_A && nB <= FP_B) {
+            for (int k = tid; k < nA; k += 256) {
+                const float yd = g.ydet[A0 + k];
+                const float x2 = c * g.OD - s * yd, y2 = s * g.OD + c * yd;
+                const float a = x2 - x1, bb = y2 - y1;
+                c_a[k] = a; c_b[k] = bb;
+                c_sba[k] = bb / a; c_sab[k] = a / bb;
+                const float l2 = a * a + bb * bb, il = 1.0f / sqrtf(l2), il2 = 1.0f / l2;
+                c_nx[k] = bb * il; c_ny[k] = -a * il;
+                c_off[k] = bb * il * x1 - a * il * y1;
+                c_hs[k] = (fl_absf(a) + fl_absf(bb)) * 0.5f * il + MISS_MARGIN;
+                c_ta[k] = a * il2; c_tb[k] = bb * il2;
+                c_t0[k] = -(x1 * a + y1 * bb) * il2;
+                c_hw[k] = (fl_absf(a) + fl_absf(bb)) * 0.5f * il2;
+            }
+            for (int k = tid; k < nB; k += 256) r_c[k] = (g.zdet[B0 + k] + sz) - z1;
+            for (int k = tid; k < nA * nB; k += 256) {
+                const int ka = k % nA, kb = k / nA;
+                const float yd = g.ydet[A0 + ka];
+                const float x2 = c * g.OD - s * yd, y2 = s * g.OD + c * yd;
+                const float a = x2 - x1, bb = y2 - y1;
+                const float cc = (g.zdet[B0 + kb] + sz) - z1;
+                q_r[k] = make_float4(cc / a, cc / bb, a / cc, bb / cc);
+                q_y[k] = fl_load(y, (size_t)(B0 + kb) * na + (A0 + ka));
+            }
+            __syncthreads();
+            if (active) {
+                const float p[3] = {x1 - xc, y1 - yc, z1 - zc};
+                const float h[3] = {0.5f, 0.5f, hz};
+                for (int ib = blo; ib <= bhi; ib++) {
+                    const int kb = ib - B0;
+                    const float cc = r_c[kb];
+                    for (int ia = alo; ia <= ahi; ia++) {
+                        const int ka = ia - A0;
+                        const float a = c_a[ka], bb = c_b[ka];
+                        float l;
+                        if (a == 0 || bb == 0) {
+                            const float yd = g.ydet[ia];
+                            l = fl_chord_box3(x1, y1, z1, c * g.OD - s * yd, s * g.OD + c * yd,
+                                              g.zdet[ib] + sz, 1.0f, 1.0f, g.dz, xc, yc, zc);
+                        } else {
+                            const float dist = c_nx[ka] * xc + c_ny[ka] * yc - c_off[ka];
+                            if (fl_absf(dist) > c_hs[ka]) continue;
+                            const float t = c_ta[ka] * xc + c_tb[ka] * yc + c_t0[ka];
+                            if (fl_absf(p[2] + cc * t) > hz + fl_absf(cc) * c_hw[ka] + MISS_MARGIN) continue;
+                            const float4 r4 = q_r[kb * nA + ka];
+                            FlRatios r;
+                            r.sba = c_sba[ka]; r.sab = c_sab[ka];
+                            r.sca = r4.x; r.scb = r4.y; r.sac = r4.z; r.sbc = r4.w;
+                            l = fl_chord_box3_r(p, h, fl_absf(a) > fl_absf(bb), r);
+                        }
+                        acc = acc + l * q_y[kb * nA + ka];
+                    }
+                }
+            }
+        } else if (active) {
+            for (int ib = blo; ib <= bhi; ib++) {
+                const float z2 = g.zdet[ib] + sz;
+                for (int ia = alo; ia <= ahi; ia++) {
+                    const float yd = g.ydet[ia];
+                    const float l = fl_chord_box3(x1, y1, z1, c * g.OD - s * yd, s * g.OD + c * yd, z2,
+                                                  1.0f, 1.0f, g.dz, xc, yc, zc);
+                    acc = acc + l * fl_load(y, (size_t)ib * na + ia);
+                }
+            }
+        }
+    }
+    if (active) fl_store(X, (size_t)it * g.nvox + ((size_t)iz * g.ny + iy) * g.nx + ix, acc * g.scale);
+}
+
 template <typename YT, typename XT>
 static hipError_t launch_adj_fan_t(const void* Y, void* X, const AdjGeom& g, hipStream_t st)
 {
@@ -119,7 +285,7 @@ static hipError_t launch_adj_cone_t(const void* Y, void* X, const AdjGeom& g, hi
 {
     uint32_t nblk = (uint32_t)((g.nx + 7) / 8) * ((g.ny + 7) / 8) * ((g.nz + 3) / 4);
     if (nblk == 0 || g.nt == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_adj_cone<YT, XT>), dim3(nblk, g.nt), dim3(256), 0, st, (const YT*)Y, (XT*)X, g);
+    hipLaunchKernelGGL((k_adj_cone_fp<YT, XT>), dim3(nblk, g.nt), dim3(256), 0, st, (const YT*)Y, (XT*)X, g);
     return hipGetLastError();
 }
 

@@ -124,3 +124,51 @@ FL_HD float fl_chord_box3(float x1, float y1, float z1, float x2, float y2, floa
     }
     return 0.0f;
 }
+
+// fl_chord_box3 with the six direction ratios supplied by the caller
+// (sba=b/a, sca=c/a, sab=a/b, scb=c/b, sac=a/c, sbc=b/c, each computed by
+// the same IEEE division the reference performs).  Identical result to
+// fl_chord_box3 for a != 0 and b != 0; the caller handles a == 0 / b == 0.
+struct FlRatios {
+    float sba, sca, sab, scb, sac, sbc;
+};
+
+template <int AX>
+FL_HD void fl_face_pair_r(const float* p, const float* h, float su, float sv, bool later, FlPts& q)
+{
+    constexpr int U = (AX == 0) ? 1 : 0;
+    constexpr int V = (AX == 2) ? 1 : 2;
+    float mu = su * (-p[AX]) + p[U], wu = su * h[AX];
+    float mv = sv * (-p[AX]) + p[V], wv = sv * h[AX];
+    if (fl_absf(mu - wu) <= h[U] && fl_absf(mv - wv) <= h[V]) {
+        float* o = q.k == 0 ? q.p0 : q.p1;
+        o[AX] = -h[AX]; o[U] = mu - wu; o[V] = mv - wv;
+        q.k++;
+    }
+    if (!later || q.k < 2) {
+        if (fl_absf(mu + wu) <= h[U] && fl_absf(mv + wv) <= h[V]) {
+            float* o = q.k == 0 ? q.p0 : q.p1;
+            o[AX] = h[AX]; o[U] = mu + wu; o[V] = mv + wv;
+            q.k++;
+        }
+    }
+}
+
+FL_HD float fl_chord_box3_r(const float* p, const float* h, bool xmajor, const FlRatios& r)
+{
+    FlPts q;
+    q.k = 0;
+    if (xmajor) {
+        fl_face_pair_r<0>(p, h, r.sba, r.sca, false, q);
+        if (q.k < 2) fl_face_pair_r<1>(p, h, r.sab, r.scb, true, q);
+    } else {
+        fl_face_pair_r<1>(p, h, r.sab, r.scb, false, q);
+        if (q.k < 2) fl_face_pair_r<0>(p, h, r.sba, r.sca, true, q);
+    }
+    if (q.k < 2) fl_face_pair_r<2>(p, h, r.sac, r.sbc, true, q);
+    if (q.k == 2) {
+        float u = q.p1[0] - q.p0[0], v = q.p1[1] - q.p0[1], w = q.p1[2] - q.p0[2];
+        return sqrtf(u * u + v * v + w * w);
+    }
+    return 0.0f;
+}
