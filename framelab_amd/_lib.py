@@ -79,7 +79,23 @@ SIGNATURES = {
     "fl_forward_gao_fan": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_int, _vp]),
     "fl_adjoint": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_int, C.c_int, _vp]),
     "fl_siddon_trace": (C.c_int, [_vp, C.c_int, _ip, C.c_int, _ip, _fp, _ip]),
+    "fl_framelet_num_bands": (C.c_int, [C.c_int, C.c_int]),
+    "fl_framelet_forward": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_int,
+                                      _vp, C.c_int, _vp]),
+    "fl_framelet_adjoint": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_int,
+                                      _vp, C.c_int, _vp, _vp]),
+    "fl_cgls": (C.c_int, [_vp, _vp, _vp, C.c_float, C.c_int, C.c_double, C.c_int,
+                          C.POINTER(C.c_double), C.POINTER(C.c_int), _vp]),
+    "fl_split_bregman": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_float, C.c_float, C.c_int, C.c_int,
+                                   C.c_double, C.c_int, C.POINTER(C.c_double), _vp]),
+    "fl_grad_data": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(C.c_double), C.c_int, _vp]),
+    "fl_grad_framelet": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_float, C.c_float,
+                                   C.c_int, _vp, _vp, C.POINTER(C.c_double), _vp]),
 }
+
+FL_FRAME_HAAR, FL_FRAME_LINEAR, FL_FRAME_CUBIC = 0, 1, 3
+FL_SOLVE_GRAPH, FL_SOLVE_MATCHED, FL_SOLVE_LOWP_BF16, FL_SOLVE_LOWP_F16 = 1, 2, 4, 8
+FL_GRAD_EXACT_ADJOINT = 1
 
 
 def lib():

@@ -222,15 +222,6 @@ extern "C" int fl_plan_create(fl_plan* out, const fl_geometry* g)
     return fl_ok();
 }
 
-void fl_plan_release_workspace(fl_plan_s* p)
-{
-    if (p && p->ws) (void)hipFree(p->ws);
-    if (p) {
-        p->ws = nullptr;
-        p->ws_bytes = 0;
-    }
-}
-
 extern "C" int fl_plan_destroy(fl_plan p)
 {
     if (!p) return fl_ok();

@@ -14,7 +14,11 @@
 
 // Storage element types of volumes and sinograms.  Arithmetic is always
 // FP32; BF16/FP16 are storage formats only (round-to-nearest-even stores).
-enum fl_dtype_t { FL_F32 = 0, FL_BF16 = 1, FL_F16 = 2 };
+#ifndef FL_F32
+#define FL_F32 0
+#define FL_BF16 1
+#define FL_F16 2
+#endif
 
 struct bf16_t { uint16_t v; };
 struct f16_t { _Float16 v; };

@@ -1,0 +1,588 @@
+// framelet.hip -- undecimated B-spline tight-framelet transforms (2D/3D) and
+// the fused split-Bregman / smoothed-l1 coefficient updates.
+//
+// Reference semantics (matlab/classes/+Transforms/@FrameletSystem):
+//   filters  GenerateFrameletFilter.m:1-30 (haar 2 filters x 3 taps, linear
+//            3 x 3, cubic 5 x 5; all boundaries circular);
+//   1D pass  utils/ConvSymAsym3D.m:19-24: circular correlation with the taps
+//            dilated by 2^(L-1), kernel centre floor((len+1)/2);
+//   2D       FraDecMultiLevel2D.m:43-51: band(i,j) = corr_y[D_j] corr_x[D_i] u;
+//            FraRecMultiLevel2D.m:203-211: u = sum_i corr_x[R_i] sum_j corr_y[R_j] C_ij;
+//   3D       FraDecMultiLevel3D.m:88-99: band(i,j,k) = corr_z[D_k] corr_x[D_j] corr_y[D_i] u;
+//            FraRecMultiLevel3D.m:246-258: u = sum_i corr_y[R_i] sum_j corr_x[R_j]
+//            sum_k corr_z[R_k] C_ijk.
+// Band b = i + NF*j (+ NF*NF*k) (MATLAB linear order of the cell {i,j,k});
+// coefficients are stored band-major: C[b][z][y][x].
+//
+// One thread per voxel; analysis gathers the NT^d neighbourhood once and
+// produces all NF^d bands from it (separably, in registers), so the input is
+// read once and every band is written once (the transform is write-bound).
+// Synthesis gathers each band's neighbourhood and accumulates in FP32.
+#include "fl_common.h"
+
+struct FlTaps {
+    float D[5][5];   // decomposition filters, NT taps each
+    float R[5][5];   // reconstruction filters
+};
+
+__device__ __forceinline__ int wrapi(int v, int n)
+{
+    v %= n;
+    return v < 0 ? v + n : v;
+}
+
+// ---------------------------------------------------------------- 3D ------
+
+template <int NF, int NT, typename XT>
+__device__ __forceinline__ void gather3(const XT* __restrict__ u, int nx, int ny, int nz, int x, int y,
+                                        int z, int step, float (&nb)[NT][NT][NT])
+{
+    constexpr int C = (NT - 1) / 2;
+    int xs[NT], ys[NT], zs[NT];
+#pragma unroll
+    for (int k = 0; k < NT; k++) {
+        xs[k] = wrapi(x + (k - C) * step, nx);
+        ys[k] = wrapi(y + (k - C) * step, ny);
+        zs[k] = wrapi(z + (k - C) * step, nz);
+    }
+#pragma unroll
+    for (int kz = 0; kz < NT; kz++)
+#pragma unroll
+        for (int ky = 0; ky < NT; ky++)
+#pragma unroll
+            for (int kx = 0; kx < NT; kx++)
+                nb[kz][ky][kx] = fl_load(u, ((size_t)zs[kz] * ny + ys[ky]) * nx + xs[kx]);
+}
+
+// All NF^3 analysis bands of one voxel from its neighbourhood:
+// band(i,j,k) = sum D_k[kz] D_j[kx] D_i[ky] nb[kz][ky][kx].
+template <int NF, int NT>
+__device__ __forceinline__ void bands3(const FlTaps& T, const float (&nb)[NT][NT][NT], float (&out)[NF * NF * NF])
+{
+    float ty[NF][NT][NT];   // corr_y with D_i: [i][kz][kx]
+#pragma unroll
+    for (int i = 0; i < NF; i++)
+#pragma unroll
+        for (int kz = 0; kz < NT; kz++)
+#pragma unroll
+            for (int kx = 0; kx < NT; kx++) {
+                float s = 0.0f;
+#pragma unroll
+                for (int ky = 0; ky < NT; ky++) s = __builtin_fmaf(T.D[i][ky], nb[kz][ky][kx], s);
+                ty[i][kz][kx] = s;
+            }
+    float tx[NF][NF][NT];   // then corr_x with D_j: [i][j][kz]
+#pragma unroll
+    for (int i = 0; i < NF; i++)
+#pragma unroll
+        for (int j = 0; j < NF; j++)
+#pragma unroll
+            for (int kz = 0; kz < NT; kz++) {
+                float s = 0.0f;
+#pragma unroll
+                for (int kx = 0; kx < NT; kx++) s = __builtin_fmaf(T.D[j][kx], ty[i][kz][kx], s);
+                tx[i][j][kz] = s;
+            }
+#pragma unroll
+    for (int i = 0; i < NF; i++)
+#pragma unroll
+        for (int j = 0; j < NF; j++)
+#pragma unroll
+            for (int k = 0; k < NF; k++) {
+                float s = 0.0f;
+#pragma unroll
+                for (int kz = 0; kz < NT; kz++) s = __builtin_fmaf(T.D[k][kz], tx[i][j][kz], s);
+                out[i + NF * j + NF * NF * k] = s;
+            }
+}
+
+template <int NF, int NT, typename XT, typename CT>
+__global__ __launch_bounds__(256) void k_fwt3(const XT* __restrict__ u, CT* __restrict__ C, int nx, int ny,
+                                             int nz, int step, FlTaps T)
+{
+    const size_t nvox = (size_t)nx * ny * nz;
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= nvox) return;
+    const int x = (int)(idx % nx), y = (int)((idx / nx) % ny), z = (int)(idx / ((size_t)nx * ny));
+    float nb[NT][NT][NT];
+    gather3<NF, NT>(u, nx, ny, nz, x, y, z, step, nb);
+    float out[NF * NF * NF];
+    bands3<NF, NT>(T, nb, out);
+#pragma unroll
+    for (int b = 0; b < NF * NF * NF; b++) fl_store(C, (size_t)b * nvox + idx, out[b]);
+}
+
+// u = alpha * sum_b corr(R-stencil of b, C_b) + beta * base (base may be null)
+// `base` may alias `u` (out = alpha*W^T C + beta*out): each thread reads and
+// writes only its own element.
+template <int NF, int NT, typename CT, typename XT>
+__global__ __launch_bounds__(256) void k_iwt3(const CT* __restrict__ C, XT* u, int nx, int ny,
+                                             int nz, int step, FlTaps T, float alpha, float beta,
+                                             const float* base)
+{
+    constexpr int CC = (NT - 1) / 2;
+    const size_t nvox = (size_t)nx * ny * nz;
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= nvox) return;
+    const int x = (int)(idx % nx), y = (int)((idx / nx) % ny), z = (int)(idx / ((size_t)nx * ny));
+    size_t off[NT][NT][NT];
+    {
+        int xs[NT], ys[NT], zs[NT];
+#pragma unroll
+        for (int k = 0; k < NT; k++) {
+            xs[k] = wrapi(x + (k - CC) * step, nx);
+            ys[k] = wrapi(y + (k - CC) * step, ny);
+            zs[k] = wrapi(z + (k - CC) * step, nz);
+        }
+#pragma unroll
+        for (int kz = 0; kz < NT; kz++)
+#pragma unroll
+            for (int ky = 0; ky < NT; ky++)
+#pragma unroll
+                for (int kx = 0; kx < NT; kx++) off[kz][ky][kx] = ((size_t)zs[kz] * ny + ys[ky]) * nx + xs[kx];
+    }
+    float acc = 0.0f;
+    for (int i = 0; i < NF; i++)
+        for (int j = 0; j < NF; j++)
+            for (int k = 0; k < NF; k++) {
+                const CT* __restrict__ cb = C + (size_t)(i + NF * j + NF * NF * k) * nvox;
+                // innermost corr_z[R_k], then corr_x[R_j], then corr_y[R_i]
+                float sy = 0.0f;
+#pragma unroll
+                for (int ky = 0; ky < NT; ky++) {
+                    float sx = 0.0f;
+#pragma unroll
+                    for (int kx = 0; kx < NT; kx++) {
+                        float sz = 0.0f;
+#pragma unroll
+                        for (int kz = 0; kz < NT; kz++) sz = __builtin_fmaf(T.R[k][kz], fl_load(cb, off[kz][ky][kx]), sz);
+                        sx = __builtin_fmaf(T.R[j][kx], sz, sx);
+                    }
+                    sy = __builtin_fmaf(T.R[i][ky], sx, sy);
+                }
+                acc += sy;
+            }
+    float v = alpha * acc;
+    if (base) v = __builtin_fmaf(beta, base[idx], v);
+    fl_store(u, idx, v);
+}
+
+// ---------------------------------------------------------------- 2D ------
+
+template <int NF, int NT, typename XT, typename CT>
+__global__ __launch_bounds__(256) void k_fwt2(const XT* __restrict__ u, CT* __restrict__ C, int nx, int ny,
+                                             int step, FlTaps T)
+{
+    constexpr int CC = (NT - 1) / 2;
+    const size_t npix = (size_t)nx * ny;
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= npix) return;
+    const int x = (int)(idx % nx), y = (int)(idx / nx);
+    float nb[NT][NT];
+#pragma unroll
+    for (int ky = 0; ky < NT; ky++) {
+        const int yy = wrapi(y + (ky - CC) * step, ny);
+#pragma unroll
+        for (int kx = 0; kx < NT; kx++) nb[ky][kx] = fl_load(u, (size_t)yy * nx + wrapi(x + (kx - CC) * step, nx));
+    }
+    float tx[NF][NT];   // corr_x with D_i: [i][ky]
+#pragma unroll
+    for (int i = 0; i < NF; i++)
+#pragma unroll
+        for (int ky = 0; ky < NT; ky++) {
+            float s = 0.0f;
+#pragma unroll
+            for (int kx = 0; kx < NT; kx++) s = __builtin_fmaf(T.D[i][kx], nb[ky][kx], s);
+            tx[i][ky] = s;
+        }
+#pragma unroll
+    for (int i = 0; i < NF; i++)
+#pragma unroll
+        for (int j = 0; j < NF; j++) {
+            float s = 0.0f;
+#pragma unroll
+            for (int ky = 0; ky < NT; ky++) s = __builtin_fmaf(T.D[j][ky], tx[i][ky], s);
+            fl_store(C, (size_t)(i + NF * j) * npix + idx, s);
+        }
+}
+
+template <int NF, int NT, typename CT, typename XT>
+__global__ __launch_bounds__(256) void k_iwt2(const CT* __restrict__ C, XT* u, int nx, int ny,
+                                             int step, FlTaps T, float alpha, float beta,
+                                             const float* base)
+{
+    constexpr int CC = (NT - 1) / 2;
+    const size_t npix = (size_t)nx * ny;
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= npix) return;
+    const int x = (int)(idx % nx), y = (int)(idx / nx);
+    int xs[NT], ys[NT];
+#pragma unroll
+    for (int k = 0; k < NT; k++) {
+        xs[k] = wrapi(x + (k - CC) * step, nx);
+        ys[k] = wrapi(y + (k - CC) * step, ny);
+    }
+    float acc = 0.0f;
+    for (int i = 0; i < NF; i++)
+        for (int j = 0; j < NF; j++) {
+            const CT* __restrict__ cb = C + (size_t)(i + NF * j) * npix;
+            float sx = 0.0f;   // corr_y[R_j] innermost, then corr_x[R_i]
+#pragma unroll
+            for (int kx = 0; kx < NT; kx++) {
+                float sy = 0.0f;
+#pragma unroll
+                for (int ky = 0; ky < NT; ky++) sy = __builtin_fmaf(T.R[j][ky], fl_load(cb, (size_t)ys[ky] * nx + xs[kx]), sy);
+                sx = __builtin_fmaf(T.R[i][kx], sy, sx);
+            }
+            acc += sx;
+        }
+    float v = alpha * acc;
+    if (base) v = __builtin_fmaf(beta, base[idx], v);
+    fl_store(u, idx, v);
+}
+
+// ----------------------------------------------- fused coefficient updates --
+//
+// Split Bregman (solveSplitBregman.m:28-36, shrink softThreshold.m:15-80):
+//   Wu = W u;  alpha = shrink(v + Wu, tau) (low-pass band copied);
+//   v <- v + (Wu - alpha);  d = alpha - v   (consumed by b = Atf + mu W^T d).
+// Analysis, shrink and both updates happen in one pass; Wu and alpha are
+// never stored.  resid accumulates per-block FP64 partials of |Wu - alpha|^2
+// (the constraint residual used as the outer stopping test).
+//
+// Smoothed l1 (gradient of lam * sum psi(Wu), psi(t) = sqrt(t^2 + eps)):
+//   g_b = lam * psi'(Wu_b) = lam * Wu_b / sqrt(Wu_b^2 + eps), low-pass band 0
+//   unless with_lowpass; loss partials of lam * sum psi.
+
+__device__ __forceinline__ float shrinkf(float t, float tau)
+{
+    float m = fabsf(t) - tau;
+    return m > 0.0f ? copysignf(m, t) : 0.0f;
+}
+
+template <int NB>
+__device__ __forceinline__ void block_reduce_store(double v, double* __restrict__ part)
+{
+    __shared__ double sred[4];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += sred[w];
+        part[blockIdx.x] = s;
+    }
+}
+
+template <int NF, int NT, typename CT>
+__global__ __launch_bounds__(256) void k_bregman3(const float* __restrict__ u, CT* __restrict__ V,
+                                                 CT* __restrict__ Dc, int nx, int ny, int nz, int step,
+                                                 FlTaps T, float tau, double* __restrict__ part)
+{
+    constexpr int NB = NF * NF * NF;
+    const size_t nvox = (size_t)nx * ny * nz;
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    double r2 = 0.0;
+    if (idx < nvox) {
+        const int x = (int)(idx % nx), y = (int)((idx / nx) % ny), z = (int)(idx / ((size_t)nx * ny));
+        float nb[NT][NT][NT];
+        gather3<NF, NT>(u, nx, ny, nz, x, y, z, step, nb);
+        float wu[NB];
+        bands3<NF, NT>(T, nb, wu);
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            const size_t o = (size_t)b * nvox + idx;
+            const float v = fl_load(V, o);
+            const float t = v + wu[b];
+            const float a = (b == 0) ? t : shrinkf(t, tau);
+            const float e = wu[b] - a;
+            const float vn = v + e;
+            r2 += (double)e * (double)e;
+            fl_store(V, o, vn);
+            fl_store(Dc, o, a - vn);
+        }
+    }
+    block_reduce_store<NB>(r2, part);
+}
+
+template <int NF, int NT, typename CT>
+__global__ __launch_bounds__(256) void k_bregman2(const float* __restrict__ u, CT* __restrict__ V,
+                                                 CT* __restrict__ Dc, int nx, int ny, int step, FlTaps T,
+                                                 float tau, double* __restrict__ part)
+{
+    constexpr int CC = (NT - 1) / 2;
+    const size_t npix = (size_t)nx * ny;
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    double r2 = 0.0;
+    if (idx < npix) {
+        const int x = (int)(idx % nx), y = (int)(idx / nx);
+        float nb[NT][NT];
+#pragma unroll
+        for (int ky = 0; ky < NT; ky++) {
+            const int yy = wrapi(y + (ky - CC) * step, ny);
+#pragma unroll
+            for (int kx = 0; kx < NT; kx++) nb[ky][kx] = u[(size_t)yy * nx + wrapi(x + (kx - CC) * step, nx)];
+        }
+#pragma unroll
+        for (int i = 0; i < NF; i++) {
+            float tx[NT];
+#pragma unroll
+            for (int ky = 0; ky < NT; ky++) {
+                float s = 0.0f;
+#pragma unroll
+                for (int kx = 0; kx < NT; kx++) s = __builtin_fmaf(T.D[i][kx], nb[ky][kx], s);
+                tx[ky] = s;
+            }
+#pragma unroll
+            for (int j = 0; j < NF; j++) {
+                float w = 0.0f;
+#pragma unroll
+                for (int ky = 0; ky < NT; ky++) w = __builtin_fmaf(T.D[j][ky], tx[ky], w);
+                const int b = i + NF * j;
+                const size_t o = (size_t)b * npix + idx;
+                const float v = fl_load(V, o);
+                const float t = v + w;
+                const float a = (b == 0) ? t : shrinkf(t, tau);
+                const float e = w - a;
+                const float vn = v + e;
+                r2 += (double)e * (double)e;
+                fl_store(V, o, vn);
+                fl_store(Dc, o, a - vn);
+            }
+        }
+    }
+    block_reduce_store<NF * NF>(r2, part);
+}
+
+template <int NF, int NT, typename CT>
+__global__ __launch_bounds__(256) void k_l1grad3(const float* __restrict__ u, CT* __restrict__ G, int nx, int ny,
+                                                int nz, int step, FlTaps T, float lam, float eps, int with_lp,
+                                                double* __restrict__ part)
+{
+    constexpr int NB = NF * NF * NF;
+    const size_t nvox = (size_t)nx * ny * nz;
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    double loss = 0.0;
+    if (idx < nvox) {
+        const int x = (int)(idx % nx), y = (int)((idx / nx) % ny), z = (int)(idx / ((size_t)nx * ny));
+        float nb[NT][NT][NT];
+        gather3<NF, NT>(u, nx, ny, nz, x, y, z, step, nb);
+        float wu[NB];
+        bands3<NF, NT>(T, nb, wu);
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            float g = 0.0f;
+            if (b != 0 || with_lp) {
+                const float r = sqrtf(__builtin_fmaf(wu[b], wu[b], eps));
+                g = lam * (wu[b] / r);
+                loss += (double)lam * (double)r;
+            }
+            fl_store(G, (size_t)b * nvox + idx, g);
+        }
+    }
+    block_reduce_store<NB>(loss, part);
+}
+
+template <int NF, int NT, typename CT>
+__global__ __launch_bounds__(256) void k_l1grad2(const float* __restrict__ u, CT* __restrict__ G, int nx, int ny,
+                                                int step, FlTaps T, float lam, float eps, int with_lp,
+                                                double* __restrict__ part)
+{
+    constexpr int CC = (NT - 1) / 2;
+    const size_t npix = (size_t)nx * ny;
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    double loss = 0.0;
+    if (idx < npix) {
+        const int x = (int)(idx % nx), y = (int)(idx / nx);
+        float nb[NT][NT];
+#pragma unroll
+        for (int ky = 0; ky < NT; ky++) {
+            const int yy = wrapi(y + (ky - CC) * step, ny);
+#pragma unroll
+            for (int kx = 0; kx < NT; kx++) nb[ky][kx] = u[(size_t)yy * nx + wrapi(x + (kx - CC) * step, nx)];
+        }
+#pragma unroll
+        for (int i = 0; i < NF; i++) {
+            float tx[NT];
+#pragma unroll
+            for (int ky = 0; ky < NT; ky++) {
+                float s = 0.0f;
+#pragma unroll
+                for (int kx = 0; kx < NT; kx++) s = __builtin_fmaf(T.D[i][kx], nb[ky][kx], s);
+                tx[ky] = s;
+            }
+#pragma unroll
+            for (int j = 0; j < NF; j++) {
+                float w = 0.0f;
+#pragma unroll
+                for (int ky = 0; ky < NT; ky++) w = __builtin_fmaf(T.D[j][ky], tx[ky], w);
+                const int b = i + NF * j;
+                float g = 0.0f;
+                if (b != 0 || with_lp) {
+                    const float r = sqrtf(__builtin_fmaf(w, w, eps));
+                    g = lam * (w / r);
+                    loss += (double)lam * (double)r;
+                }
+                fl_store(G, (size_t)b * npix + idx, g);
+            }
+        }
+    }
+    block_reduce_store<NF * NF>(loss, part);
+}
+
+// --------------------------------------------------------------- host side --
+
+#include "fl_framelet.h"
+
+static FlTaps make_taps(int kind)
+{
+    FlTaps T = {};
+    const double s2 = 1.4142135623730951, s6 = 2.449489742783178;
+    if (kind == FL_FRAME_HAAR) {
+        const double D[2][3] = {{0, 0.5, 0.5}, {0, -0.5, 0.5}};
+        const double R[2][3] = {{0.5, 0.5, 0}, {0.5, -0.5, 0}};
+        for (int i = 0; i < 2; i++)
+            for (int k = 0; k < 3; k++) { T.D[i][k] = (float)D[i][k]; T.R[i][k] = (float)R[i][k]; }
+    } else if (kind == FL_FRAME_LINEAR) {
+        const double D[3][3] = {{0.25, 0.5, 0.25}, {-s2 / 4, 0, s2 / 4}, {-0.25, 0.5, -0.25}};
+        const double R[3][3] = {{0.25, 0.5, 0.25}, {s2 / 4, 0, -s2 / 4}, {-0.25, 0.5, -0.25}};
+        for (int i = 0; i < 3; i++)
+            for (int k = 0; k < 3; k++) { T.D[i][k] = (float)D[i][k]; T.R[i][k] = (float)R[i][k]; }
+    } else {
+        const double D[5][5] = {{1 / 16., 4 / 16., 6 / 16., 4 / 16., 1 / 16.},
+                                {-1 / 8., -2 / 8., 0, 2 / 8., 1 / 8.},
+                                {-s6 / 16, 0, 2 * s6 / 16, 0, -s6 / 16},
+                                {-1 / 8., 2 / 8., 0, -2 / 8., 1 / 8.},
+                                {-1 / 16., 4 / 16., -6 / 16., 4 / 16., -1 / 16.}};
+        const double R[5][5] = {{1 / 16., 4 / 16., 6 / 16., 4 / 16., 1 / 16.},
+                                {1 / 8., 2 / 8., 0, -2 / 8., -1 / 8.},
+                                {-s6 / 16, 0, 2 * s6 / 16, 0, -s6 / 16},
+                                {1 / 8., -2 / 8., 0, 2 / 8., -1 / 8.},
+                                {-1 / 16., 4 / 16., -6 / 16., 4 / 16., -1 / 16.}};
+        for (int i = 0; i < 5; i++)
+            for (int k = 0; k < 5; k++) { T.D[i][k] = (float)D[i][k]; T.R[i][k] = (float)R[i][k]; }
+    }
+    return T;
+}
+
+int fl_frame_nf(int kind) { return kind == FL_FRAME_HAAR ? 2 : kind == FL_FRAME_LINEAR ? 3 : 5; }
+
+static unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
+
+#define FL_FRAME_SWITCH(kind, CALL)                                                   \
+    do {                                                                              \
+        if (kind == FL_FRAME_HAAR) { constexpr int NF = 2, NT = 3; CALL; }            \
+        else if (kind == FL_FRAME_LINEAR) { constexpr int NF = 3, NT = 3; CALL; }     \
+        else { constexpr int NF = 5, NT = 5; CALL; }                                  \
+    } while (0)
+
+template <typename XT, typename CT>
+static hipError_t fwt_t(const FlFrameDesc& f, const void* u, void* C, int level, hipStream_t st)
+{
+    const FlTaps T = make_taps(f.kind);
+    const int step = 1 << (level - 1);
+    const size_t n = (size_t)f.nx * f.ny * (f.dim == 3 ? f.nz : 1);
+    if (f.dim == 3)
+        FL_FRAME_SWITCH(f.kind, hipLaunchKernelGGL((k_fwt3<NF, NT, XT, CT>), dim3(nblk(n)), dim3(256), 0, st,
+                                                   (const XT*)u, (CT*)C, f.nx, f.ny, f.nz, step, T));
+    else
+        FL_FRAME_SWITCH(f.kind, hipLaunchKernelGGL((k_fwt2<NF, NT, XT, CT>), dim3(nblk(n)), dim3(256), 0, st,
+                                                   (const XT*)u, (CT*)C, f.nx, f.ny, step, T));
+    return hipGetLastError();
+}
+
+template <typename CT, typename XT>
+static hipError_t iwt_t(const FlFrameDesc& f, const void* C, void* u, int level, float alpha, float beta,
+                        const float* base, hipStream_t st)
+{
+    const FlTaps T = make_taps(f.kind);
+    const int step = 1 << (level - 1);
+    const size_t n = (size_t)f.nx * f.ny * (f.dim == 3 ? f.nz : 1);
+    if (f.dim == 3)
+        FL_FRAME_SWITCH(f.kind, hipLaunchKernelGGL((k_iwt3<NF, NT, CT, XT>), dim3(nblk(n)), dim3(256), 0, st,
+                                                   (const CT*)C, (XT*)u, f.nx, f.ny, f.nz, step, T, alpha,
+                                                   beta, base));
+    else
+        FL_FRAME_SWITCH(f.kind, hipLaunchKernelGGL((k_iwt2<NF, NT, CT, XT>), dim3(nblk(n)), dim3(256), 0, st,
+                                                   (const CT*)C, (XT*)u, f.nx, f.ny, step, T, alpha, beta,
+                                                   base));
+    return hipGetLastError();
+}
+
+hipError_t fl_launch_fwt(const FlFrameDesc& f, const void* u, int ut, void* C, int ct, int level, hipStream_t st)
+{
+    if (ut == FL_F32 && ct == FL_F32) return fwt_t<float, float>(f, u, C, level, st);
+    if (ut == FL_F32 && ct == FL_BF16) return fwt_t<float, bf16_t>(f, u, C, level, st);
+    if (ut == FL_F32 && ct == FL_F16) return fwt_t<float, f16_t>(f, u, C, level, st);
+    if (ut == FL_BF16 && ct == FL_BF16) return fwt_t<bf16_t, bf16_t>(f, u, C, level, st);
+    if (ut == FL_F16 && ct == FL_F16) return fwt_t<f16_t, f16_t>(f, u, C, level, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t fl_launch_iwt(const FlFrameDesc& f, const void* C, int ct, void* u, int ut, int level, float alpha,
+                         float beta, const float* base, hipStream_t st)
+{
+    if (ct == FL_F32 && ut == FL_F32) return iwt_t<float, float>(f, C, u, level, alpha, beta, base, st);
+    if (ct == FL_BF16 && ut == FL_F32) return iwt_t<bf16_t, float>(f, C, u, level, alpha, beta, base, st);
+    if (ct == FL_F16 && ut == FL_F32) return iwt_t<f16_t, float>(f, C, u, level, alpha, beta, base, st);
+    if (ct == FL_BF16 && ut == FL_BF16) return iwt_t<bf16_t, bf16_t>(f, C, u, level, alpha, beta, base, st);
+    if (ct == FL_F16 && ut == FL_F16) return iwt_t<f16_t, f16_t>(f, C, u, level, alpha, beta, base, st);
+    return hipErrorInvalidValue;
+}
+
+template <typename CT>
+static hipError_t bregman_t(const FlFrameDesc& f, const float* u, void* V, void* D, float tau, double* part,
+                            hipStream_t st)
+{
+    const FlTaps T = make_taps(f.kind);
+    const size_t n = (size_t)f.nx * f.ny * (f.dim == 3 ? f.nz : 1);
+    if (f.dim == 3)
+        FL_FRAME_SWITCH(f.kind, hipLaunchKernelGGL((k_bregman3<NF, NT, CT>), dim3(nblk(n)), dim3(256), 0, st, u,
+                                                   (CT*)V, (CT*)D, f.nx, f.ny, f.nz, 1, T, tau, part));
+    else
+        FL_FRAME_SWITCH(f.kind, hipLaunchKernelGGL((k_bregman2<NF, NT, CT>), dim3(nblk(n)), dim3(256), 0, st, u,
+                                                   (CT*)V, (CT*)D, f.nx, f.ny, 1, T, tau, part));
+    return hipGetLastError();
+}
+
+hipError_t fl_launch_bregman(const FlFrameDesc& f, const float* u, void* V, void* D, int ct, float tau,
+                             double* part, hipStream_t st)
+{
+    if (ct == FL_F32) return bregman_t<float>(f, u, V, D, tau, part, st);
+    if (ct == FL_BF16) return bregman_t<bf16_t>(f, u, V, D, tau, part, st);
+    if (ct == FL_F16) return bregman_t<f16_t>(f, u, V, D, tau, part, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t fl_launch_l1grad(const FlFrameDesc& f, const float* u, float* G, float lam, float eps, int with_lp,
+                            double* part, hipStream_t st)
+{
+    const FlTaps T = make_taps(f.kind);
+    const size_t n = (size_t)f.nx * f.ny * (f.dim == 3 ? f.nz : 1);
+    if (f.dim == 3)
+        FL_FRAME_SWITCH(f.kind, hipLaunchKernelGGL((k_l1grad3<NF, NT, float>), dim3(nblk(n)), dim3(256), 0, st, u,
+                                                   G, f.nx, f.ny, f.nz, 1, T, lam, eps, with_lp, part));
+    else
+        FL_FRAME_SWITCH(f.kind, hipLaunchKernelGGL((k_l1grad2<NF, NT, float>), dim3(nblk(n)), dim3(256), 0, st, u,
+                                                   G, f.nx, f.ny, 1, T, lam, eps, with_lp, part));
+    return hipGetLastError();
+}
+
+// copy of one band (the low-pass of a level) into a volume buffer
+__global__ void k_copy_f32(const float* __restrict__ a, float* __restrict__ b, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+hipError_t fl_launch_copy(const float* a, float* b, size_t n, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy_f32, dim3(nblk(n)), dim3(256), 0, st, a, b, n);
+    return hipGetLastError();
+}
+
+unsigned fl_frame_blocks(const FlFrameDesc& f)
+{
+    return nblk((size_t)f.nx * f.ny * (f.dim == 3 ? f.nz : 1));
+}

@@ -1,0 +1,67 @@
+"""Vector-space data types used by the operators and solvers.
+
+Mirrors matlab/classes/+DataTypes/ObjectData.m:1-129 (image/volume with its
+physical extent; objdot/norm in FP64 here, single precision in the
+reference) and @CTData/CTData.m:1-87 (sinogram plus its scanner).  The data
+lives in a GPU tensor; shapes follow the projector layouts ([nz,]ny,nx and
+nv,[nb,]na).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+
+class ObjectData:
+    def __init__(self, dim: int, dataArray, L=None):
+        if not isinstance(dataArray, torch.Tensor):
+            dataArray = torch.as_tensor(np.ascontiguousarray(dataArray, dtype=np.float32))
+        self.dim = dim
+        self.dataArray = dataArray
+        shape = tuple(dataArray.shape[-dim:])
+        self.N = tuple(reversed(shape))                 # (nx, ny[, nz]) as MATLAB N
+        self.L = tuple(L) if L is not None else tuple(2.0 for _ in range(dim))
+        self.Dx = tuple(l / n for l, n in zip(self.L, self.N))
+
+    def _like(self, a):
+        return ObjectData(self.dim, a, self.L)
+
+    def _check(self, o):
+        if self.dim != o.dim or self.L != o.L:
+            raise ValueError("Either dimensions or sizes are wrong!")
+
+    def __add__(self, o):
+        self._check(o)
+        return self._like(self.dataArray + o.dataArray)
+
+    def __sub__(self, o):
+        self._check(o)
+        return self._like(self.dataArray - o.dataArray)
+
+    def __neg__(self):
+        return self._like(-self.dataArray)
+
+    def __rmul__(self, a):
+        return self._like(a * self.dataArray)
+
+    __mul__ = __rmul__
+
+    def objdot(self, o) -> float:
+        return float(torch.dot(self.dataArray.reshape(-1).double(), o.dataArray.reshape(-1).double()))
+
+    def norm(self) -> float:
+        return float(np.sqrt(self.objdot(self)))
+
+
+class CTData:
+    """Sinogram data with the scanner that produced it (CTData.m:1-87)."""
+
+    def __init__(self, scanner, dataArray, dataArrayNorm=None, L=None):
+        self.scanner = scanner
+        self.dataArray = dataArray
+        self.dataArrayNorm = dataArrayNorm
+        self.L = L
+        self.para = getattr(scanner, "para", None)
+
+    def objdot(self, o) -> float:
+        return float(torch.dot(self.dataArray.reshape(-1).double(), o.dataArray.reshape(-1).double()))

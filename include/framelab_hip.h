@@ -127,6 +127,73 @@ int fl_forward_gao_fan(fl_plan plan, const void *d_x, int x_dtype, void *d_y, in
 int fl_adjoint(fl_plan plan, const void *d_y, int y_dtype, void *d_x, int x_dtype, int kind,
                void *stream);
 
+/* ------------------------------------------------ framelet transforms */
+
+/* Frame codes of GenerateFrameletFilter
+ * (matlab/classes/+Transforms/@FrameletSystem/GenerateFrameletFilter.m:1-30). */
+#define FL_FRAME_HAAR 0
+#define FL_FRAME_LINEAR 1
+#define FL_FRAME_CUBIC 3
+
+/* Bands per level: NF^dim (NF = 2, 3, 5 filters), or a negative status. */
+int fl_framelet_num_bands(int dim, int kind);
+
+/* Analysis W (FrameletSystem.forwardTransform, FrameletSystem.m:31-49 ->
+ * FraDecMultiLevel{2D,3D}.m).  u: nz*ny*nx (nz = 1 for dim 2); c: `level`
+ * blocks of NB band volumes, band b = i + NF*j (+ NF^2*k), level L using
+ * dilation 2^(L-1) on the low-pass band of level L-1. */
+int fl_framelet_forward(int dim, int kind, int level, int nx, int ny, int nz, const void *d_u,
+                        int u_dtype, void *d_c, int c_dtype, void *stream);
+
+/* Synthesis W^T (FrameletSystem.adjointTransform, FrameletSystem.m:51-64 ->
+ * FraRecMultiLevel{2D,3D}.m).  For level > 1 the coefficients must be FP32
+ * and d_work must hold 2*NB*nz*ny*nx floats; d_c is not modified. */
+int fl_framelet_adjoint(int dim, int kind, int level, int nx, int ny, int nz, const void *d_c,
+                        int c_dtype, void *d_u, int u_dtype, void *d_work, void *stream);
+
+/* ------------------------------------------------------------- solvers */
+
+#define FL_SOLVE_GRAPH 1     /* capture one iteration in a hipGraph and replay it */
+#define FL_SOLVE_MATCHED 2   /* use the matched (Siddon-transpose) adjoint */
+#define FL_SOLVE_LOWP_BF16 4 /* projector input and sinogram stored in BF16 */
+#define FL_SOLVE_LOWP_F16 8  /* ... or in FP16 */
+
+/* CGLS: CG on (A^T A + lam I) u = b.  Replaces CGLS.solveCGLS
+ * (matlab/classes/+Optimizers/@CGLS/solveCGLS.m:1-31).  d_u holds u0 on entry
+ * and the iterate on return.  FP64 alpha/beta/gamma on the device; the loop
+ * stops once gamma = r.r < tol.  gamma_log (iters+1 doubles) and iters_done
+ * are optional host outputs (requesting them synchronises). */
+int fl_cgls(fl_plan plan, const float *d_b, float *d_u, float lam, int iters, double tol,
+            int flags, double *gamma_log, int *iters_done, void *stream);
+
+/* Split Bregman with the framelet analysis model.  Replaces
+ * SplitBregman.solveSplitBregman (matlab/classes/+Optimizers/@SplitBregman/
+ * solveSplitBregman.m:1-44): outer iterations of an `inner`-step CGLS with
+ * lam = mu, shrink(v + Wu, lam/mu) (low-pass kept), v <- v + (Wu - alpha).
+ * d_f: sinogram; d_u: u0 in, u out.  resid_log (outer doubles, optional):
+ * |Wu - alpha|^2 after each outer step. */
+int fl_split_bregman(fl_plan plan, int frame_kind, const float *d_f, float *d_u, float lam,
+                     float mu, int outer, int inner, double cg_tol, int flags, double *resid_log,
+                     void *stream);
+
+/* ----------------------------------------------------------- gradients */
+
+#define FL_GRAD_EXACT_ADJOINT 1 /* use the exact-chord adjoint instead of the matched one */
+
+/* g = A^T (A x - b) and loss = 0.5 |A x - b|^2 (loss optional; FP64).  No
+ * reference counterpart: the backward of the forward projector. */
+int fl_grad_data(fl_plan plan, const float *d_x, const float *d_b, float *d_g, double *loss,
+                 int flags, void *stream);
+
+/* g += W^T (lam * psi'(W x)) with psi(t) = sqrt(t^2 + eps) (smoothed l1),
+ * loss = lam * sum psi (optional); one level; the low-pass band is left
+ * unpenalised unless with_lowpass.  d_work: NB*nz*ny*nx floats.  d_g is
+ * accumulated into (pass zeros for the framelet term alone).  No reference
+ * counterpart: the backward of the analysis transform. */
+int fl_grad_framelet(int dim, int kind, int nx, int ny, int nz, const float *d_x, float lam,
+                     float eps, int with_lowpass, float *d_g, float *d_work, double *loss,
+                     void *stream);
+
 /* Traversal log of selected rays (testing): rays = (view, ia, ib) triples
  * (host), cap entries per ray; J/S/cnt host outputs of nrays*cap / nrays. */
 int fl_siddon_trace(fl_plan plan, int nrays, const int *rays, int cap, int *J, float *S, int *cnt);

@@ -311,10 +311,10 @@ static double ray_f64(int nx, int ny, int nz, double dz0,
     return acc;
 }
 
-/* FP64 cone (nz>0) or fan (nz==0, nb==1, z ignored) forward.
- * Geometry in double: phi -> cos/sin in double. */
+/* FP64 cone (nz>0) or fan (nz==0, nb==1, z ignored) forward; per-view
+ * cos/sin of the view angle are inputs. */
 void orc_siddon_f64(const double *X, double *y, double SO, double OD, double scale,
-                    int nx, int ny, int nz, int nv, const double *phi, const double *sd_z,
+                    int nx, int ny, int nz, int nv, const double *cphi, const double *sphi, const double *sd_z,
                     int na, int nb, const double *y_det, const double *z_det, double dz0)
 {
     size_t nd = (size_t)na * nb;
@@ -329,7 +329,7 @@ void orc_siddon_f64(const double *X, double *y, double SO, double OD, double sca
 #pragma omp for schedule(dynamic, 4)
         for (int r = 0; r < rays; r++) {
             int iv = r / nb, ib = r % nb;
-            double c = cos(phi[iv]), s = sin(phi[iv]);
+            double c = cphi[iv], s = sphi[iv];
             double zs = nz > 0 ? sd_z[iv] : 0.0;
             for (int ia = 0; ia < na; ia++) {
                 double x1 = -SO * c, y1 = -SO * s;
@@ -346,7 +346,7 @@ void orc_siddon_f64(const double *X, double *y, double SO, double OD, double sca
 /* FP64 exact transpose of orc_siddon_f64 (ray-driven scatter).  Runs
  * serially so the summation order is fixed. */
 void orc_siddon_adj_f64(double *X, const double *y, double SO, double OD, double scale,
-                        int nx, int ny, int nz, int nv, const double *phi, const double *sd_z,
+                        int nx, int ny, int nz, int nv, const double *cphi, const double *sphi, const double *sd_z,
                         int na, int nb, const double *y_det, const double *z_det, double dz0)
 {
     size_t nvox = (size_t)nx * ny * (nz > 0 ? nz : 1), nd = (size_t)na * nb;
@@ -357,7 +357,7 @@ void orc_siddon_adj_f64(double *X, const double *y, double SO, double OD, double
     double *al = malloc((nx + ny + nz + 3) * sizeof(double));
     memset(X, 0, nvox * sizeof(double));
     for (int iv = 0; iv < nv; iv++) {
-        double c = cos(phi[iv]), s = sin(phi[iv]);
+        double c = cphi[iv], s = sphi[iv];
         double zs = nz > 0 ? sd_z[iv] : 0.0;
         for (int ib = 0; ib < nb; ib++)
             for (int ia = 0; ia < na; ia++) {

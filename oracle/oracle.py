@@ -66,7 +66,7 @@ def lib():
         _lib.orc_siddon_trace_f32.restype = _i
         _lib.orc_siddon_trace_f32.argtypes = [_i, _ip, _f, _f, _i, _i, _i, _fp, _fp, _fp, _fp, _fp,
                                               _f, _i, _ip, _fp, _ip]
-        _lib.orc_siddon_f64.argtypes = [_dp, _dp, _d, _d, _d, _i, _i, _i, _i, _dp, _dp, _i, _i,
+        _lib.orc_siddon_f64.argtypes = [_dp, _dp, _d, _d, _d, _i, _i, _i, _i, _dp, _dp, _dp, _i, _i,
                                         _dp, _dp, _d]
         _lib.orc_siddon_adj_f64.argtypes = _lib.orc_siddon_f64.argtypes
         _lib.orc_find_l.restype = _f
@@ -188,40 +188,49 @@ def siddon_trace(p, rays, cone=True):
     return [(J[off[k]:off[k + 1]].copy(), S[off[k]:off[k + 1]].copy()) for k in range(n)]
 
 
-def _geom64(p, cone):
+def _geom64(p, cone, trig="f64"):
+    """FP64 geometry.  trig='f64': cos/sin of the angle in double; 'f32':
+    the float cosf/sinf values the FP32 forward uses, promoted (same rays as
+    the FP32 path, exact arithmetic along them)."""
+    if trig == "f32":
+        c, s = trig_f32(p.sd_phi)
+        cphi, sphi = c.astype(np.float64), s.astype(np.float64)
+    else:
+        phi = np.asarray(p.sd_phi, dtype=np.float64)
+        cphi, sphi = np.cos(phi), np.sin(phi)
     phi = np.asarray(p.sd_phi, dtype=np.float64)
     sdz = np.asarray(p.sd_z, dtype=np.float64) if cone else np.zeros(phi.size)
     ydet = np.asarray(p.y_det, dtype=np.float64)
     zdet = np.asarray(p.z_det, dtype=np.float64) if cone else np.zeros(1)
-    return phi, sdz, ydet, zdet
+    return (cphi, sphi), sdz, ydet, zdet
 
 
-def siddon_f64(X, p, cone=True):
+def siddon_f64(X, p, cone=True, trig="f64"):
     """FP64 Siddon forward (same traversal rule in double; accuracy oracle).
 
     Pinned by closed forms (box/disc/ball chords) and brute-force clipping."""
-    phi, sdz, ydet, zdet = _geom64(p, cone)
+    (cphi, sphi), sdz, ydet, zdet = _geom64(p, cone, trig)
     nz = p.nz if cone else 0
     nb = p.nb if cone else 1
     na = p.na if cone else p.nd
     X = np.ascontiguousarray(X, dtype=np.float64).ravel()
-    y = np.zeros(phi.size * na * nb)
+    y = np.zeros(cphi.size * na * nb)
     lib().orc_siddon_f64(_pd(X), _pd(y), float(p.SO), float(p.OD), float(p.scale), p.nx, p.ny,
-                         nz, phi.size, _pd(phi), _pd(sdz), na, nb, _pd(ydet), _pd(zdet),
+                         nz, cphi.size, _pd(cphi), _pd(sphi), _pd(sdz), na, nb, _pd(ydet), _pd(zdet),
                          float(p.dz) if cone else 1.0)
     return y
 
 
-def siddon_adj_f64(y, p, cone=True):
+def siddon_adj_f64(y, p, cone=True, trig="f64"):
     """FP64 exact transpose of siddon_f64 (dot-product test to ~1e-15)."""
-    phi, sdz, ydet, zdet = _geom64(p, cone)
+    (cphi, sphi), sdz, ydet, zdet = _geom64(p, cone, trig)
     nz = p.nz if cone else 0
     nb = p.nb if cone else 1
     na = p.na if cone else p.nd
     y = np.ascontiguousarray(y, dtype=np.float64).ravel()
     X = np.zeros(p.nx * p.ny * max(nz, 1))
     lib().orc_siddon_adj_f64(_pd(X), _pd(y), float(p.SO), float(p.OD), float(p.scale), p.nx,
-                             p.ny, nz, phi.size, _pd(phi), _pd(sdz), na, nb, _pd(ydet),
+                             p.ny, nz, cphi.size, _pd(cphi), _pd(sphi), _pd(sdz), na, nb, _pd(ydet),
                              _pd(zdet), float(p.dz) if cone else 1.0)
     return X
 
@@ -477,7 +486,10 @@ def cgls(A, At, b, u0, lam, iters, tol=0.0, log=None):
     gold = float(np.dot(r.ravel(), r.ravel()))
     for _ in range(iters):
         q = Aloc(p)
-        alpha = gold / float(np.dot(p.ravel(), q.ravel()))
+        pq = float(np.dot(p.ravel(), q.ravel()))
+        if pq == 0.0:          # exact convergence (p = 0): nothing left to do
+            break
+        alpha = gold / pq
         u = u + alpha * p
         r = r + alpha * q
         gnew = float(np.dot(r.ravel(), r.ravel()))
