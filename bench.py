@@ -8,7 +8,17 @@ forward projection A (Siddon) plus one backprojection A^T (exact
 voxel-driven adjoint) of the whole volume, inputs resident in HBM.
 
 Prints ONE JSON line.  value = proj+backproj GUPS = 2*N_vox*n_views /
-(t_A + t_At) / 1e9 over the timed steps.
+(t_A + t_At) / 1e9 over the timed steps (FP32 plain path).  Beside it:
+  cgls      -- CGLS iterations/s of the FP32 plain path (Siddon + exact
+               adjoint, FP32 storage) and of the mixed path (FP16 storage of
+               the projector input and sinogram, FP32 accumulation, matched
+               adjoint), both device-resident with FP64 reductions and
+               graph-replayed iterations, and the mixed reconstruction's
+               relative L2 difference to the FP32 one;
+  roofline  -- the dominant kernel's algorithmic bytes per launch over its
+               measured duration, against 8 TB/s;
+  cpu_baseline -- the reference's own single-threaded CPU kernels on a
+               bounded sample of views.
 """
 from __future__ import annotations
 
@@ -47,7 +57,6 @@ def cpu_baseline(p, X, views=3):
     """The reference's own single-threaded CPU kernels (oracle/_ref), or the
     oracle's restatement when _ref is absent, on `views` views of the same
     volume: Siddon forward + exact adjoint, reported as proj+backproj GUPS."""
-    import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from framelab_amd import geometry as G
@@ -72,12 +81,31 @@ def cpu_baseline(p, X, views=3):
                       f"(t_A={t1 - t0:.2f}s, t_At={t2 - t1:.2f}s); host: {model}, {cores} logical CPUs"}
 
 
+def time_cgls(P, b, u0, iters, **kw):
+    """Seconds per CGLS iteration (the init's normal-operator pass excluded)."""
+    import torch
+    from framelab_amd.solvers import cgls
+    cgls(P, b, u0, 0.0, 1, log=False, **kw)                  # warm: workspace + graph capture
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cgls(P, b, u0, 0.0, 0, log=False, **kw)
+    torch.cuda.synchronize()
+    t_init = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    u, _, _ = cgls(P, b, u0, 0.0, iters, log=False, **kw)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    return (t - t_init) / iters, u
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cgls-iters", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cgls", action="store_true", help="skip the CGLS leg")
     ap.add_argument("--cpu-views", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -133,7 +161,7 @@ def main():
 
     # per-operator algorithmic bytes (per-view streaming model): nv*(Nvox*sx + Ndet*sy)
     B_op = nv * (nvox * 4 + ndet * 4)
-    dom_ms, dom_name = (mAt, "k_adj_cone<float,float>") if mAt >= mA else (mA, "k_fwd_cone<float,float,64,4>")
+    dom_ms, dom_name = (mAt, "k_adj_cone_fp<float,float>") if mAt >= mA else (mA, "k_fwd_cone<float,float,64,4>")
     achieved = B_op / (dom_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom_name,
@@ -142,7 +170,7 @@ def main():
     # matched adjoint and low-precision storage variants (reported beside)
     extra = {"t_A_ms": mA, "t_At_exact_ms": mAt}
     evm = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    step(kind=ADJ_MATCHED)
+    P.adjoint(y, out=xb, kind=ADJ_MATCHED)
     evm[0].record()
     P.adjoint(y, out=xb, kind=ADJ_MATCHED)
     evm[1].record()
@@ -154,12 +182,12 @@ def main():
         xo = torch.empty(P.vol_shape, dtype=dt, device=dev)
         step(xin=xl, yb=yl, xo=xo)
         e3 = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-        step(xin=xl, yb=yl, xo=xo, evs=e3)
+        step(xin=xl, yb=yl, xo=xo, evs=e3, kind=ADJ_MATCHED)
         torch.cuda.synchronize()
         a, b = e3[0].elapsed_time(e3[1]), e3[1].elapsed_time(e3[2])
         extra[f"{name}_t_A_ms"] = a
-        extra[f"{name}_t_At_ms"] = b
-        extra[f"{name}_gups"] = 2.0 * nvox * nv / ((a + b) * 1e-3) / 1e9
+        extra[f"{name}_t_At_matched_ms"] = b
+        extra[f"{name}_matched_gups"] = 2.0 * nvox * nv / ((a + b) * 1e-3) / 1e9
 
     result = {
         "metric": METRIC, "value": gups, "unit": "GUPS", "n_gpus": 1, "steps": args.steps,
@@ -172,6 +200,32 @@ def main():
         "roofline": roof,
         "extra": extra,
     }
+
+    if not args.no_cgls:
+        log("bench: cgls")
+        b_rhs = P.adjoint(P.forward(x))
+        u0 = torch.zeros_like(x)
+        it = args.cgls_iters
+        s32, u32 = time_cgls(P, b_rhs, u0, it, graph=True)
+        bm = P.adjoint(P.forward(x), kind=ADJ_MATCHED)
+        smx, umx = time_cgls(P, bm, u0, it, graph=True, matched=True, storage=torch.float16)
+        s16, u16 = time_cgls(P, b_rhs, u0, it, graph=True, storage=torch.float16)
+        rel = float((umx - u32).norm() / u32.norm())
+        rel16 = float((u16 - u32).norm() / u32.norm())
+        result["cgls"] = {
+            "iters": it,
+            "fp32_plain_iters_per_s": 1.0 / s32,
+            "mixed_iters_per_s": 1.0 / smx,
+            "mixed_vs_fp32_speedup": s32 / smx,
+            "mixed_rel_l2_vs_fp32": rel,
+            "fp16_exact_adjoint_iters_per_s": 1.0 / s16,
+            "fp16_exact_adjoint_rel_l2_vs_fp32": rel16,
+            "fp32_plain": "FP32 storage, Siddon + exact adjoint, FP64 dots, graph-replayed",
+            "mixed": "FP16 storage of p and A p, FP32 accumulation, Siddon + matched adjoint, FP64 dots, "
+                     "graph-replayed",
+        }
+        log(f"bench: cgls fp32 {1 / s32:.2f} it/s, mixed {1 / smx:.2f} it/s, rel {rel:.2e}")
+
     if not args.no_cpu:
         log("bench: cpu baseline")
         result["cpu_baseline"] = cpu_baseline(p, Xh, views=args.cpu_views)

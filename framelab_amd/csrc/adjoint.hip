@@ -181,6 +181,13 @@ __global__ __launch_bounds__(256) void k_adj_cone_fp(const YT* __restrict__ Y, X
             if (ahi > na - 1) ahi = na - 1;
             if (blo < 0) blo = 0;
             if (bhi > nb - 1) bhi = nb - 1;
+            // Narrow to the cells whose ray centre meets the voxel's shadow
+            // (plus a twentieth of a cell): every cell outside it has a zero
+            // chord, so the sum over the intersection is the reference sum.
+            const ShadowWin sw = fl_shadow_window(c, s, g.SO, SD, xc, yc, zr, hz, g.ya0, g.inv_pa, g.zb0,
+                                                  g.inv_pb, 0.05f, na, nb);
+            alo = max(alo, sw.alo); ahi = min(ahi, sw.ahi);
+            blo = max(blo, sw.blo); bhi = min(bhi, sw.bhi);
             if (alo > ahi || blo > bhi) { alo = na; ahi = -1; blo = nb; bhi = -1; }
         }
         // workgroup footprint = union of the voxels' windows

@@ -208,6 +208,24 @@ extern "C" int fl_plan_create(fl_plan* out, const fl_geometry* g)
     a.dz_det = g->dz_det; a.dz = cone ? g->dz : 1.0f;
     a.nx = g->nx; a.ny = g->ny; a.nz = cone ? g->nz : 1; a.na = g->na; a.nb = g->nb; a.nt = g->nt;
     a.tmp_size = tmp; a.nvox = (size_t)nvox; a.nd = (size_t)ndet;
+    // detector index maps (uniform pitch) used to bound voxel shadows
+    {
+        const double pa = g->na > 1 ? ((double)g->y_det[g->na - 1] - g->y_det[0]) / (g->na - 1) : (double)g->dy_det;
+        const double pb = (cone && g->nb > 1) ? ((double)g->z_det[g->nb - 1] - g->z_det[0]) / (g->nb - 1)
+                                              : (double)(cone ? g->dz_det : 1.0f);
+        f.ya0 = a.ya0 = g->y_det[0];
+        f.inv_pa = a.inv_pa = (float)(1.0 / pa);
+        f.zb0 = a.zb0 = cone ? g->z_det[0] : 0.0f;
+        f.inv_pb = a.inv_pb = (float)(1.0 / pb);
+        const double rvol = 0.5 * sqrt((double)g->nx * g->nx + (double)g->ny * g->ny) + 1.0;
+        if (!(pa != 0.0 && pb != 0.0) || !(g->SO > rvol)) {
+            plan_free_arrays(p);
+            delete p;
+            return fl_fail(FL_ERR_INVALID, "source must lie outside the volume (SO=%g voxel units, volume "
+                                           "radius %g) and detector pitches must be non-zero",
+                           (double)g->SO, rvol);
+        }
+    }
     if (cone && !(g->dy_det > 0 && g->dz_det > 0)) {
         plan_free_arrays(p);
         delete p;

@@ -172,3 +172,52 @@ FL_HD float fl_chord_box3_r(const float* p, const float* h, bool xmajor, const F
     }
     return 0.0f;
 }
+
+// Detector cells whose ray can meet a voxel box, from the projection of the
+// box corners (shadow bounding box) widened by `margin` cells.  Rotated
+// frame: xr along the source->centre axis, yr along the detector row; the
+// ray of column a passes the detector at u = y_det[a] and meets a point
+// (xr, yr) when u = yr*SD/(xr+SO) (likewise rows: v = z_det[b] and
+// zr*SD/(xr+SO), zr relative to the source height).  Corner offsets of the
+// rotated unit square are (c*dx+s*dy, -s*dx+c*dy), dx,dy = +-1/2.
+// Hardware reciprocals are enough here: the margin (a twentieth of a cell)
+// dwarfs their rounding.
+struct ShadowWin {
+    int alo, ahi, blo, bhi;
+};
+
+__device__ __forceinline__ ShadowWin fl_shadow_window(float c, float s, float SO, float SD, float xc, float yc,
+                                                     float zr, float hz, float ya0, float inv_pa, float zb0,
+                                                     float inv_pb, float margin, int na, int nb)
+{
+    const float xr = c * xc + s * yc, yr = -s * xc + c * yc;
+    const float hc = 0.5f * c, hs = 0.5f * s;
+    float umin = FL_INF, umax = -FL_INF;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float ox = (k & 1) ? hc : -hc, oy = (k & 2) ? hs : -hs;   // c*dx, s*dy
+        const float xk = xr + ox + oy;
+        const float yk = yr + ((k & 1) ? -hs : hs) + ((k & 2) ? hc : -hc);
+        const float u = yk * SD * __builtin_amdgcn_rcpf(xk + SO);
+        umin = fminf(umin, u);
+        umax = fmaxf(umax, u);
+    }
+    const float ex = 0.5f * (fabsf(c) + fabsf(s));
+    const float rn = SD * __builtin_amdgcn_rcpf(xr - ex + SO), rf = SD * __builtin_amdgcn_rcpf(xr + ex + SO);
+    const float zl = zr - hz, zh = zr + hz;
+    const float vmax = zh > 0 ? zh * rn : zh * rf;
+    const float vmin = zl < 0 ? zl * rn : zl * rf;
+    ShadowWin w;
+    // (a pitch may be negative: order the mapped ends)
+    const float a1 = (umin - ya0) * inv_pa, a2 = (umax - ya0) * inv_pa;
+    const float b1 = (vmin - zb0) * inv_pb, b2 = (vmax - zb0) * inv_pb;
+    w.alo = (int)ceilf(fminf(a1, a2) - margin);
+    w.ahi = (int)floorf(fmaxf(a1, a2) + margin);
+    w.blo = (int)ceilf(fminf(b1, b2) - margin);
+    w.bhi = (int)floorf(fmaxf(b1, b2) + margin);
+    if (w.alo < 0) w.alo = 0;
+    if (w.ahi > na - 1) w.ahi = na - 1;
+    if (w.blo < 0) w.blo = 0;
+    if (w.bhi > nb - 1) w.bhi = nb - 1;
+    return w;
+}

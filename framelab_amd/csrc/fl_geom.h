@@ -15,6 +15,8 @@ struct FwdGeom {
     float SO, OD, scale, dz0;
     int nx, ny, nz, na, nb, nslots;
     size_t nvox;            // voxels per frame
+    // detector index maps for voxel shadows: cell a is at y_det[0] + a*pitch
+    float ya0, inv_pa, zb0, inv_pb;
 };
 
 // Voxel-driven adjoint geometry (the reference para tables).
@@ -32,4 +34,5 @@ struct AdjGeom {
     int nx, ny, nz, na, nb, nt, tmp_size;
     size_t nvox;
     size_t nd;              // detector cells per view
+    float ya0, inv_pa, zb0, inv_pb;
 };

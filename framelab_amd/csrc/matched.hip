@@ -21,6 +21,7 @@
 #include "fl_common.h"
 #include "fl_geom.h"
 #include "fl_siddon.h"
+#include "fl_chord.h"
 
 namespace {
 
@@ -45,27 +46,69 @@ __device__ __forceinline__ AxisIv slab(const SidAxis& a, int cell)
 
 }  // namespace
 
-// 3D: one thread per voxel in 8x8x4 bricks; views of each frame in order.
+// Per (ray, voxel) pair of one view: the Siddon segment of the ray inside
+// the voxel, credited when the forward's midpoint rule maps it there.
+// in/out: entry/exit plane parameters per axis.  Returns the length
+// coefficient (a_out - a_in)*|d|, or 0.
+__device__ __forceinline__ float matched_coef(float inx, float outx, float iny, float outy, float inz, float outz,
+                                              const SidAxis& ax, const SidAxis& ay, const SidAxis& az, int ix,
+                                              int iy, int iz, bool dz_one, float L)
+{
+    const float ain = fmaxf(fmaxf(inx, iny), inz), aout = fminf(fminf(outx, outy), outz);
+    if (!(aout > ain)) return 0.0f;
+    const float m = (ain + aout) * 0.5f;
+    if (sid_cell_xy(ax, m) != ix || sid_cell_xy(ay, m) != iy || sid_cell_z(az, m, dz_one) != iz) return 0.0f;
+    return (aout - ain) * L;
+}
+
+namespace {
+constexpr int MF_A = 48;   // footprint columns in LDS
+constexpr int MF_B = 24;   // footprint rows
+
+__device__ __forceinline__ int mwave_min(int v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int mwave_max(int v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+}  // namespace
+
+// 3D, footprint-staged: per view the workgroup (an 8x8x4 voxel brick) stages
+// in LDS the plane parameters of its nine x planes and nine y planes for
+// each footprint column, of its five z planes for each footprint row (each
+// the forward's IEEE division, done once per workgroup), the ray lengths
+// |d| and the sinogram values; each voxel then visits only the cells of its
+// own shadow.  Views of each frame in order; FP32 accumulation.
 template <typename YT, typename XT>
 __global__ __launch_bounds__(256) void k_adj_matched_cone(const YT* __restrict__ Y, XT* __restrict__ X,
                                                          FwdGeom f, AdjGeom g)
 {
+    __shared__ float t_ax[MF_A][9], t_ay[MF_A][9];
+    __shared__ float t_dx[MF_A], t_dy[MF_A];
+    __shared__ float t_az[MF_B][5], t_dz[MF_B];
+    __shared__ float t_L[MF_A * MF_B], t_y[MF_A * MF_B];
+    __shared__ int red[2][4][4];
+
     const int bx = (f.nx + 7) / 8, by = (f.ny + 7) / 8, bz = (f.nz + 3) / 4;
     const uint32_t nblk = (uint32_t)bx * by * bz;
     const uint32_t b = fl_xcd_remap(blockIdx.x, nblk);
     const int it = blockIdx.y;
-    const int ix = (int)(b % bx) * 8 + (threadIdx.x & 7);
-    const int iy = (int)((b / bx) % by) * 8 + ((threadIdx.x >> 3) & 7);
-    const int iz = (int)(b / ((uint32_t)bx * by)) * 4 + (threadIdx.x >> 6);
-    if (ix >= f.nx || iy >= f.ny || iz >= f.nz) return;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int ix0 = (int)(b % bx) * 8, iy0 = (int)((b / bx) % by) * 8, iz0 = (int)(b / ((uint32_t)bx * by)) * 4;
+    const int lx = tid & 7, ly = (tid >> 3) & 7, lz = tid >> 6;
+    const int ix = ix0 + lx, iy = iy0 + ly, iz = iz0 + lz;
+    const bool active = ix < f.nx && iy < f.ny && iz < f.nz;
 
     const int na = f.na, nb = f.nb;
     const float SD = f.SO + f.OD;
     const float xc = (float)ix + 0.5f - (float)(f.nx / 2);
     const float yc = (float)iy + 0.5f - (float)(f.ny / 2);
     const float zc = ((float)iz + 0.5f - (float)(f.nz / 2)) * f.dz0;
-    const float ya0 = f.ydet[0], zb0 = f.zdet[0];
-    const float hz = 0.5f * f.dz0 + 0.5f;   // z half-extent plus a cell of margin (voxel units)
+    const float hz = 0.5f * f.dz0;
     const bool dz_one = f.dz0 == 1.0f;
     float acc = 0.0f;
     const int nvf = g.Nv[it];
@@ -74,52 +117,105 @@ __global__ __launch_bounds__(256) void k_adj_matched_cone(const YT* __restrict__
         const YT* __restrict__ y = Y + (size_t)iv * g.nd;
         const float c = f.cs[iv], s = f.sn[iv];
         const float sz = f.sdz[iv];
-        // detector window of the voxel's shadow, with a margin of one cell
-        const float xr = c * xc + s * yc, yr = -s * xc + c * yc;
-        const float mag = SD / (xr + f.SO);
-        int alo = (int)floorf(((yr - 1.0f) * mag - ya0) / g.dy_det) - 1;
-        int ahi = (int)ceilf(((yr + 1.0f) * mag - ya0) / g.dy_det) + 1;
-        const float zr = zc - sz;
-        const float mz = SD / (xr + f.SO - 0.75f);
-        const float zlo = fminf((zr - hz) * mz, (zr - hz) * mag), zhi = fmaxf((zr + hz) * mz, (zr + hz) * mag);
-        int blo = (int)floorf((zlo - zb0) / g.dz_det) - 1;
-        int bhi = (int)ceilf((zhi - zb0) / g.dz_det) + 1;
-        if (alo < 0) alo = 0;
-        if (ahi > na - 1) ahi = na - 1;
-        if (blo < 0) blo = 0;
-        if (bhi > nb - 1) bhi = nb - 1;
-
+        int alo = na, ahi = -1, blo = nb, bhi = -1;
+        if (active) {
+            const ShadowWin w = fl_shadow_window(c, s, f.SO, SD, xc, yc, zc - sz, hz, f.ya0, f.inv_pa, f.zb0,
+                                                 f.inv_pb, 0.05f, na, nb);
+            if (w.alo <= w.ahi && w.blo <= w.bhi) { alo = w.alo; ahi = w.ahi; blo = w.blo; bhi = w.bhi; }
+        }
+        {
+            int m0 = mwave_min(alo), m1 = mwave_max(ahi), m2 = mwave_min(blo), m3 = mwave_max(bhi);
+            if (lane == 0) {
+                red[j & 1][wv][0] = m0; red[j & 1][wv][1] = m1;
+                red[j & 1][wv][2] = m2; red[j & 1][wv][3] = m3;
+            }
+        }
+        __syncthreads();
+        int A0 = red[j & 1][0][0], A1 = red[j & 1][0][1], B0 = red[j & 1][0][2], B1 = red[j & 1][0][3];
+#pragma unroll
+        for (int w = 1; w < 4; w++) {
+            A0 = min(A0, red[j & 1][w][0]); A1 = max(A1, red[j & 1][w][1]);
+            B0 = min(B0, red[j & 1][w][2]); B1 = max(B1, red[j & 1][w][3]);
+        }
+        const int nA = A1 - A0 + 1, nB = B1 - B0 + 1;
+        if (nA <= 0 || nB <= 0) continue;
         const float x1 = c * (-f.SO), y1 = s * (-f.SO), z1 = sz;
-        for (int ia = alo; ia <= ahi; ia++) {
-            const float yd = f.ydet[ia];
-            const float x2 = c * f.OD - s * yd;
-            const float y2 = s * f.OD + c * yd;
-            const float dx = x2 - x1, dy = y2 - y1;
-            SidAxis ax, ay;
-            sid_axis_init(ax, x1, dx, 1.0f, f.nx);
-            sid_axis_init(ay, y1, dy, 1.0f, f.ny);
-            const AxisIv sx = slab(ax, ix), sy = slab(ay, iy);
-            const float in_xy = fmaxf(sx.in, sy.in), out_xy = fminf(sx.out, sy.out);
-            if (!(out_xy > in_xy)) continue;
-            const float dxy2 = dx * dx + dy * dy;
+        if (nA <= MF_A && nB <= MF_B) {
+            // columns: direction components and the brick's x/y plane parameters
+            for (int k = tid; k < nA * 9; k += 256) {
+                const int ka = k / 9, q = k % 9;
+                const float yd = f.ydet[A0 + ka];
+                const float dx = (c * f.OD - s * yd) - x1, dy = (s * f.OD + c * yd) - y1;
+                // plane coordinate i -> ((float)i - o)/d, as the forward computes it
+                t_ax[ka][q] = ((float)(ix0 + q - f.nx / 2) - x1) / dx;
+                t_ay[ka][q] = ((float)(iy0 + q - f.ny / 2) - y1) / dy;
+                if (q == 0) { t_dx[ka] = dx; t_dy[ka] = dy; }
+            }
+            for (int k = tid; k < nB * 5; k += 256) {
+                const int kb = k / 5, q = k % 5;
+                const float dz = (f.zdet[B0 + kb] + sz) - z1;
+                t_az[kb][q] = ((float)(iz0 + q - f.nz / 2) * f.dz0 - z1) / dz;
+                if (q == 0) t_dz[kb] = dz;
+            }
+            for (int k = tid; k < nA * nB; k += 256) {
+                const int ka = k % nA, kb = k / nA;
+                const float yd = f.ydet[A0 + ka];
+                const float dx = (c * f.OD - s * yd) - x1, dy = (s * f.OD + c * yd) - y1;
+                const float dz = (f.zdet[B0 + kb] + sz) - z1;
+                t_L[k] = sqrtf(dx * dx + dy * dy + dz * dz);
+                t_y[k] = fl_load(y, (size_t)(B0 + kb) * na + (A0 + ka));
+            }
+            __syncthreads();
+            if (active && alo <= ahi) {
+                for (int ib = blo; ib <= bhi; ib++) {
+                    const int kb = ib - B0;
+                    const float dz = t_dz[kb];
+                    SidAxis az;
+                    sid_axis_init(az, z1, dz, f.dz0, f.nz);
+                    const float z_lo = t_az[kb][lz], z_hi = t_az[kb][lz + 1];
+                    const float inz = az.on ? (dz > 0 ? z_lo : z_hi) : -FL_INF;
+                    const float outz = az.on ? (dz > 0 ? z_hi : z_lo) : FL_INF;
+                    for (int ia = alo; ia <= ahi; ia++) {
+                        const int ka = ia - A0;
+                        const float dx = t_dx[ka], dy = t_dy[ka];
+                        SidAxis ax, ay;
+                        sid_axis_init(ax, x1, dx, 1.0f, f.nx);
+                        sid_axis_init(ay, y1, dy, 1.0f, f.ny);
+                        const float x_lo = t_ax[ka][lx], x_hi = t_ax[ka][lx + 1];
+                        const float y_lo = t_ay[ka][ly], y_hi = t_ay[ka][ly + 1];
+                        const float inx = ax.on ? (dx > 0 ? x_lo : x_hi) : -FL_INF;
+                        const float outx = ax.on ? (dx > 0 ? x_hi : x_lo) : FL_INF;
+                        const float iny = ay.on ? (dy > 0 ? y_lo : y_hi) : -FL_INF;
+                        const float outy = ay.on ? (dy > 0 ? y_hi : y_lo) : FL_INF;
+                        const int k = kb * nA + ka;
+                        const float Si = matched_coef(inx, outx, iny, outy, inz, outz, ax, ay, az, ix, iy, iz,
+                                                      dz_one, t_L[k]);
+                        if (Si != 0.0f) acc = acc + Si * t_y[k];
+                    }
+                }
+            }
+        } else if (active && alo <= ahi) {
             for (int ib = blo; ib <= bhi; ib++) {
-                const float z2 = f.zdet[ib] + sz;
-                const float dz = z2 - z1;
+                const float dz = (f.zdet[ib] + sz) - z1;
                 SidAxis az;
                 sid_axis_init(az, z1, dz, f.dz0, f.nz);
                 const AxisIv sv = slab(az, iz);
-                const float ain = fmaxf(in_xy, sv.in), aout = fminf(out_xy, sv.out);
-                if (!(aout > ain)) continue;
-                const float m = (ain + aout) * 0.5f;
-                if (sid_cell_xy(ax, m) != ix || sid_cell_xy(ay, m) != iy || sid_cell_z(az, m, dz_one) != iz)
-                    continue;
-                const float L = sqrtf(dxy2 + dz * dz);
-                const float Si = (aout - ain) * L;
-                acc = acc + Si * fl_load(y, (size_t)ib * na + ia);
+                for (int ia = alo; ia <= ahi; ia++) {
+                    const float yd = f.ydet[ia];
+                    const float dx = (c * f.OD - s * yd) - x1, dy = (s * f.OD + c * yd) - y1;
+                    SidAxis ax, ay;
+                    sid_axis_init(ax, x1, dx, 1.0f, f.nx);
+                    sid_axis_init(ay, y1, dy, 1.0f, f.ny);
+                    const AxisIv sx = slab(ax, ix), sy = slab(ay, iy);
+                    const float L = sqrtf(dx * dx + dy * dy + dz * dz);
+                    const float Si = matched_coef(sx.in, sx.out, sy.in, sy.out, sv.in, sv.out, ax, ay, az, ix, iy,
+                                                  iz, dz_one, L);
+                    if (Si != 0.0f) acc = acc + Si * fl_load(y, (size_t)ib * na + ia);
+                }
             }
         }
     }
-    fl_store(X, (size_t)it * f.nvox + ((size_t)iz * f.ny + iy) * f.nx + ix, acc * f.scale);
+    if (active) fl_store(X, (size_t)it * f.nvox + ((size_t)iz * f.ny + iy) * f.nx + ix, acc * f.scale);
 }
 
 template <typename YT, typename XT>
