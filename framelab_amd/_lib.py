@@ -91,6 +91,16 @@ SIGNATURES = {
     "fl_grad_data": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(C.c_double), C.c_int, _vp]),
     "fl_grad_framelet": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_float, C.c_float,
                                    C.c_int, _vp, _vp, C.POINTER(C.c_double), _vp]),
+    "fl_batch_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.POINTER(FlGeometry)]),
+    "fl_batch_destroy": (C.c_int, [_vp]),
+    "fl_batch_layout": (C.c_int, [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "fl_batch_forward": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
+    "fl_batch_adjoint": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
+    "fl_batch_cgls": (C.c_int, [_vp, _vp, _vp, C.c_float, C.c_int, C.c_double, C.c_int,
+                                C.POINTER(C.c_double), C.POINTER(C.c_int), _vp]),
+    "fl_batch_split_bregman": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_float, C.c_float, C.c_int, C.c_int,
+                                         C.c_double, C.c_int, C.POINTER(C.c_double), _vp]),
+    "fl_batch_autotune": (C.c_int, [_vp, _ip, _ip, _ip]),
 }
 
 FL_FRAME_HAAR, FL_FRAME_LINEAR, FL_FRAME_CUBIC = 0, 1, 3

@@ -12,6 +12,7 @@
 #include "fl_chord.h"
 #include "fl_common.h"
 #include "fl_geom.h"
+#include "fl_fan.h"
 
 template <typename YT, typename XT>
 __global__ __launch_bounds__(256) void k_adj_fan(const YT* __restrict__ Y, XT* __restrict__ X, AdjGeom g)
@@ -20,33 +21,7 @@ __global__ __launch_bounds__(256) void k_adj_fan(const YT* __restrict__ Y, XT* _
     const int iy = blockIdx.y * 16 + threadIdx.y;
     const int it = blockIdx.z;
     if (ix >= g.nx || iy >= g.ny) return;
-    const int nd = g.na, nd2 = nd / 2;
-    const float SD = g.SO + g.OD;
-    const float xc = (float)ix + 0.5f - (float)(g.nx / 2);
-    const float yc = (float)iy + 0.5f - (float)(g.ny / 2);
-    float acc = 0.0f;
-    const int nvf = g.Nv[it];
-    for (int j = 0; j < nvf; j++) {
-        const int iv = g.id_Y[it * g.tmp_size + j];
-        const YT* __restrict__ y = Y + (size_t)iv * nd;
-        const float c = g.cphi[iv], s = g.sphi[iv];
-        const float xr = c * xc + s * yc;
-        const float yr = -s * xc + c * yc;
-        const float t = SD / ((xr + g.SO) * g.dy_det);
-        int hi = (int)floorf((yr + 1) * t - g.y_os + (float)nd2);
-        int lo = (int)floorf((yr - 1) * t - g.y_os + (float)nd2);
-        if (lo < 0) lo = 0;
-        if (hi > nd - 1) hi = nd - 1;
-        for (int id = lo; id <= hi; id++) {
-            const float sd = g.sdet[id], cd = g.cdet[id];
-            const float Px = -(sd * c + cd * s);
-            const float Py = cd * c - sd * s;
-            const float Pd = g.SO * sd;
-            const float l = fl_chord_line2(Px, Py, Pd, xc, yc);
-            if (l > 0) acc = acc + l * fl_load(y, (size_t)id);
-        }
-    }
-    fl_store(X, (size_t)it * g.nvox + (size_t)iy * g.nx + ix, acc * g.scale);
+    fl_store(X, (size_t)it * g.nvox + (size_t)iy * g.nx + ix, fl_fan_adj_pixel(Y, g, it, ix, iy));
 }
 
 // Cone: a workgroup is an 8x8x4 voxel brick, so the 256 threads gather from

@@ -1,0 +1,775 @@
+// batch.hip -- many independent 2D fan-beam problems of uneven sizes in one
+// persistent launch per operator.
+//
+// Every problem keeps its own geometry tables (a plan); volumes and
+// sinograms are packed back to back.  Each operator is one launch of a
+// persistent grid that strides over a flat list of work items -- ray tiles
+// (forward), 8x8 pixel tiles (adjoint) or 4096-element chunks (vector
+// updates) -- mapped to their problem by binary search over per-problem
+// prefix counts.  Items are assigned statically (item = block + k*grid), so
+// results are deterministic and identical to the single-problem kernels
+// (the per-ray / per-pixel bodies are shared, fl_fan.h).  CGLS and split
+// Bregman run all problems in lock step with per-problem FP64 reductions
+// and stopping flags; one iteration of the whole batch is one hipGraph.
+// The ray-tile width and the persistent grid sizes are picked by timing the
+// candidates on the batch itself (fl_batch_autotune).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "../../include/framelab_hip.h"
+#include "fl_capi_internal.h"
+#include "fl_fan.h"
+#include "fl_framelet.h"
+#include "fl_framelet_dev.h"
+
+namespace {
+
+constexpr int CH = 4096;   // elements per vector work item
+
+struct BDesc {
+    FwdGeom f;
+    AdjGeom a;
+    long long voff, soff;    // packed offsets of the problem's volume / sinogram
+    int n;                   // pixels
+    int fwd0, adj0, vec0;    // first work item of each kind
+};
+
+struct BState {
+    double gamma, alpha, beta, tol;
+    int stop, iter;
+};
+
+__device__ __forceinline__ int find_problem(const int* __restrict__ start, int P, int item)
+{
+    int lo = 0, hi = P;     // start[lo] <= item < start[hi]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (start[mid] <= item) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------- operators
+
+template <bool GAO>
+__global__ __launch_bounds__(64) void k_bfwd(const float* __restrict__ X, float* __restrict__ Y,
+                                            const BDesc* __restrict__ D, const int* __restrict__ start, int P,
+                                            int T, int total)
+{
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const int p = find_problem(start, P, item);
+        const FwdGeom f = D[p].f;
+        const long long voff = D[p].voff, soff = D[p].soff;
+        const int local = item - start[p];
+        const int tpv = (f.na + T - 1) / T;
+        const int slot = local / tpv, t = local % tpv;
+        const int iv = f.slot_view[slot];
+        const float* x = X + voff + (size_t)f.view_frame[iv] * f.nvox;
+        for (int k = threadIdx.x; k < T; k += 64) {
+            const int id = t * T + k;
+            if (id < f.na) {
+                const float v = GAO ? fl_fan_gao_sum(x, f, iv, id) : fl_fan_ray_sum(x, f, iv, id);
+                Y[soff + (size_t)slot * f.na + id] = v;
+            }
+        }
+    }
+}
+
+template <bool MATCHED>
+__global__ __launch_bounds__(64) void k_badj(const float* __restrict__ Y, float* __restrict__ X,
+                                            const BDesc* __restrict__ D, const int* __restrict__ start, int P,
+                                            int total)
+{
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const int p = find_problem(start, P, item);
+        const int local = item - start[p];
+        const int nx = D[p].f.nx, ny = D[p].f.ny;
+        const int tx = (nx + 7) / 8;
+        const int ix = (local % tx) * 8 + (threadIdx.x & 7), iy = (local / tx) * 8 + (threadIdx.x >> 3);
+        if (ix < nx && iy < ny) {
+            const float* y = Y + D[p].soff;
+            const float v = MATCHED ? fl_fan_matched_pixel(y, D[p].f, D[p].a, 0, ix, iy)
+                                    : fl_fan_adj_pixel(y, D[p].a, 0, ix, iy);
+            X[D[p].voff + (size_t)iy * nx + ix] = v;
+        }
+    }
+}
+
+// ------------------------------------------------------- vector updates ---
+
+__device__ __forceinline__ void item_partial(double v, double* __restrict__ part, int item)
+{
+    __shared__ double s[4];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) part[item] = (s[0] + s[1]) + (s[2] + s[3]);
+    __syncthreads();
+}
+
+// chunk `c` of problem p: [voff + c*CH, voff + min(n, (c+1)*CH))
+#define BATCH_VEC_LOOP(body)                                                                  \
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {                            \
+        const int p = find_problem(start, P, item);                                           \
+        const int c = item - start[p];                                                        \
+        const long long base = D[p].voff;                                                     \
+        const int n = D[p].n;                                                                 \
+        const int hi = min(n, (c + 1) * CH);                                                  \
+        double acc = 0.0;                                                                     \
+        const bool run = !(st && st[p].stop);                                                 \
+        if (run)                                                                              \
+            for (int e = c * CH + (int)threadIdx.x; e < hi; e += 256) {                       \
+                const size_t i = (size_t)(base + e);                                          \
+                body                                                                          \
+            }                                                                                 \
+        if (part) item_partial(acc, part, item);                                              \
+    }
+
+__global__ __launch_bounds__(256) void k_bv_init(const float* __restrict__ q, const float* __restrict__ u,
+                                                const float* __restrict__ b, float lam, float* __restrict__ r,
+                                                float* __restrict__ pv, double* __restrict__ part,
+                                                const BDesc* __restrict__ D, const int* __restrict__ start, int P,
+                                                int total)
+{
+    const BState* st = nullptr;
+    BATCH_VEC_LOOP({
+        float ri = __builtin_fmaf(lam, u[i], q[i]) - b[i];
+        r[i] = ri;
+        pv[i] = -ri;
+        acc += (double)ri * (double)ri;
+    })
+}
+
+__global__ __launch_bounds__(256) void k_bv_pq(float* __restrict__ q, const float* __restrict__ pv, float lam,
+                                              double* __restrict__ part, const BState* __restrict__ st,
+                                              const BDesc* __restrict__ D, const int* __restrict__ start, int P,
+                                              int total)
+{
+    BATCH_VEC_LOOP({
+        float qi = __builtin_fmaf(lam, pv[i], q[i]);
+        q[i] = qi;
+        acc += (double)pv[i] * (double)qi;
+    })
+}
+
+__global__ __launch_bounds__(256) void k_bv_update(float* __restrict__ u, float* __restrict__ r,
+                                                  const float* __restrict__ pv, const float* __restrict__ q,
+                                                  double* __restrict__ part, const BState* __restrict__ st,
+                                                  const BDesc* __restrict__ D, const int* __restrict__ start,
+                                                  int P, int total)
+{
+    BATCH_VEC_LOOP({
+        const float a = (float)st[p].alpha;
+        u[i] = __builtin_fmaf(a, pv[i], u[i]);
+        float ri = __builtin_fmaf(a, q[i], r[i]);
+        r[i] = ri;
+        acc += (double)ri * (double)ri;
+    })
+}
+
+__global__ __launch_bounds__(256) void k_bv_p(float* __restrict__ pv, const float* __restrict__ r,
+                                             const BState* __restrict__ st, const BDesc* __restrict__ D,
+                                             const int* __restrict__ start, int P, int total)
+{
+    double* part = nullptr;
+    BATCH_VEC_LOOP({
+        pv[i] = __builtin_fmaf((float)st[p].beta, pv[i], -r[i]);
+        (void)acc;
+    })
+}
+
+__global__ void k_bv_copy(const float* __restrict__ a, float* __restrict__ b, long long n)
+{
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) b[i] = a[i];
+}
+
+// one workgroup per problem: fixed-order sum of its items' partials
+__device__ __forceinline__ double sum_items(const double* __restrict__ part, int i0, int i1)
+{
+    __shared__ double s[256];
+    double v = 0.0;
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) v += part[i];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    return s[0];
+}
+
+__global__ __launch_bounds__(256) void k_bfin_gamma(const double* __restrict__ part, const int* __restrict__ start,
+                                                   BState* st, double tol, double* __restrict__ log, int maxlog)
+{
+    const int p = blockIdx.x;
+    const double g = sum_items(part, start[p], start[p + 1]);
+    if (threadIdx.x == 0) {
+        st[p].gamma = g; st[p].alpha = 0.0; st[p].beta = 0.0; st[p].tol = tol;
+        st[p].iter = 0;
+        st[p].stop = g < tol ? 1 : 0;
+        if (log) log[(size_t)p * maxlog] = g;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bfin_alpha(const double* __restrict__ part, const int* __restrict__ start,
+                                                   BState* st)
+{
+    const int p = blockIdx.x;
+    const double pq = sum_items(part, start[p], start[p + 1]);
+    if (threadIdx.x == 0 && !st[p].stop) {
+        if (pq == 0.0) { st[p].stop = 1; st[p].alpha = 0.0; }
+        else st[p].alpha = st[p].gamma / pq;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bfin_beta(const double* __restrict__ part, const int* __restrict__ start,
+                                                  BState* st, double* __restrict__ log, int maxlog)
+{
+    const int p = blockIdx.x;
+    const double g = sum_items(part, start[p], start[p + 1]);
+    if (threadIdx.x == 0 && !st[p].stop) {
+        st[p].beta = g / st[p].gamma;
+        st[p].gamma = g;
+        st[p].iter += 1;
+        if (log && st[p].iter < maxlog) log[(size_t)p * maxlog + st[p].iter] = g;
+        if (g < st[p].tol) st[p].stop = 1;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bfin_sum(const double* __restrict__ part, const int* __restrict__ start,
+                                                 double* __restrict__ out, int stride, int k)
+{
+    const int p = blockIdx.x;
+    const double v = sum_items(part, start[p], start[p + 1]);
+    if (threadIdx.x == 0) out[(size_t)p * stride + k] = v;
+}
+
+// --------------------------------------------------- batched framelets ----
+
+template <int NF, int NT>
+__global__ __launch_bounds__(256) void k_bfwt2(const float* __restrict__ u, float* __restrict__ C, FlTaps T,
+                                              const BDesc* __restrict__ D, const int* __restrict__ start, int P,
+                                              int total)
+{
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const int p = find_problem(start, P, item);
+        const int c = item - start[p];
+        const int nx = D[p].f.nx, ny = D[p].f.ny, n = D[p].n;
+        const float* up = u + D[p].voff;
+        float* Cp = C + (size_t)NF * NF * D[p].voff;
+        for (int e = c * CH + (int)threadIdx.x; e < min(n, (c + 1) * CH); e += 256) {
+            float nb[NT][NT];
+            fl_gather2<NT>(up, nx, ny, e % nx, e / nx, 1, nb);
+            float w[NF * NF];
+            fl_bands2<NF, NT>(T, nb, w);
+#pragma unroll
+            for (int b = 0; b < NF * NF; b++) Cp[(size_t)b * n + e] = w[b];
+        }
+    }
+}
+
+template <int NF, int NT>
+__global__ __launch_bounds__(256) void k_bbregman2(const float* __restrict__ u, float* __restrict__ V,
+                                                  float* __restrict__ Dc, FlTaps T, float tau,
+                                                  double* __restrict__ part, const BDesc* __restrict__ D,
+                                                  const int* __restrict__ start, int P, int total)
+{
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const int p = find_problem(start, P, item);
+        const int c = item - start[p];
+        const int nx = D[p].f.nx, ny = D[p].f.ny, n = D[p].n;
+        const size_t co = (size_t)NF * NF * D[p].voff;
+        double acc = 0.0;
+        for (int e = c * CH + (int)threadIdx.x; e < min(n, (c + 1) * CH); e += 256)
+            acc += fl_bregman2_pixel<NF, NT>(u + D[p].voff, V + co, Dc + co, n, e, nx, ny, e % nx, e / nx, 1, T, tau);
+        item_partial(acc, part, item);
+    }
+}
+
+// out = alpha * W^T C + beta * base
+template <int NF, int NT>
+__global__ __launch_bounds__(256) void k_biwt2(const float* __restrict__ C, float* out, float alpha, float beta,
+                                              const float* base, FlTaps T, const BDesc* __restrict__ D,
+                                              const int* __restrict__ start, int P, int total)
+{
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const int p = find_problem(start, P, item);
+        const int c = item - start[p];
+        const int nx = D[p].f.nx, ny = D[p].f.ny, n = D[p].n;
+        const size_t co = (size_t)NF * NF * D[p].voff;
+        for (int e = c * CH + (int)threadIdx.x; e < min(n, (c + 1) * CH); e += 256) {
+            float v = alpha * fl_synth2<NF, NT>(C + co, n, nx, ny, e % nx, e / nx, 1, T);
+            const size_t i = (size_t)(D[p].voff + e);
+            if (base) v = __builtin_fmaf(beta, base[i], v);
+            out[i] = v;
+        }
+    }
+}
+
+}  // namespace
+
+// ================================================================ host ====
+
+struct fl_batch_s {
+    int P = 0;
+    int device = 0;
+    int cus = 256;
+    std::vector<fl_plan> plans;
+    std::vector<BDesc> desc;
+    std::vector<long long> voff, soff;   // P+1 each
+    BDesc* d_desc = nullptr;
+    int *d_fstart = nullptr, *d_astart = nullptr, *d_vstart = nullptr;
+    int fwd_total = 0, adj_total = 0, vec_total = 0;
+    int T = 64;                          // rays per forward item
+    int fwd_grid = 0, adj_grid = 0, vec_grid = 0;
+    // solver workspace
+    float *r = nullptr, *p = nullptr, *q = nullptr, *sino = nullptr;
+    double* part = nullptr;
+    BState* st = nullptr;
+    double* log = nullptr;
+    int maxlog = 0;
+    float *V = nullptr, *Dc = nullptr, *Atf = nullptr, *b = nullptr;
+    int nbands = 0;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    const void* gkey = nullptr;
+    float gkey_lam = 0.0f;
+    int gkey_flags = -1;
+    // solvers run on a private stream ordered after / before the caller's
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    std::mutex mu;
+};
+
+static void b_drop_graph(fl_batch_s* B)
+{
+    if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    B->gexec = nullptr;
+    B->graph = nullptr;
+    B->gkey = nullptr;
+}
+
+#define BCHK(expr)                                            \
+    do {                                                      \
+        hipError_t e_ = (expr);                               \
+        if (e_ != hipSuccess) return fl_hip_fail(e_, #expr);  \
+    } while (0)
+
+static int b_rebuild_fwd_table(fl_batch_s* B)
+{
+    std::vector<int> fs(B->P + 1, 0);
+    for (int i = 0; i < B->P; i++) {
+        const FwdGeom& f = B->desc[i].f;
+        fs[i + 1] = fs[i] + f.nslots * ((f.na + B->T - 1) / B->T);
+        B->desc[i].fwd0 = fs[i];
+    }
+    B->fwd_total = fs[B->P];
+    BCHK(hipMemcpy(B->d_fstart, fs.data(), fs.size() * sizeof(int), hipMemcpyHostToDevice));
+    BCHK(hipMemcpy(B->d_desc, B->desc.data(), B->desc.size() * sizeof(BDesc), hipMemcpyHostToDevice));
+    return FL_OK;
+}
+
+static void b_free(fl_batch_s* B)
+{
+    for (fl_plan pl : B->plans) fl_plan_destroy(pl);
+    void* ptrs[] = {B->d_desc, B->d_fstart, B->d_astart, B->d_vstart, B->r, B->p, B->q, B->sino, B->part,
+                    B->st, B->log, B->V, B->Dc, B->Atf, B->b};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    b_drop_graph(B);
+    if (B->ev_in) (void)hipEventDestroy(B->ev_in);
+    if (B->ev_out) (void)hipEventDestroy(B->ev_out);
+    if (B->stream) (void)hipStreamDestroy(B->stream);
+}
+
+extern "C" int fl_batch_create(fl_batch* out, int nprob, const fl_geometry* geoms)
+{
+    if (!out || nprob <= 0 || !geoms) return fl_fail(FL_ERR_INVALID, "bad batch arguments");
+    *out = nullptr;
+    long long nv_total = 0, ns_total = 0;
+    for (int i = 0; i < nprob; i++) {   // checked before any device call
+        const fl_geometry& g = geoms[i];
+        if (g.nz != 0 || g.nt != 1)
+            return fl_fail(FL_ERR_INVALID, "batch problem %d: fan geometries with one frame only", i);
+        nv_total += (long long)g.nx * g.ny;
+        ns_total += (long long)g.na * g.nv;
+    }
+    if (nv_total >= (1LL << 31) || ns_total >= (1LL << 31))
+        return fl_fail(FL_ERR_INVALID, "batch too large: %lld pixels, %lld detector cells (limit 2^31)",
+                       nv_total, ns_total);
+    fl_batch_s* B = new (std::nothrow) fl_batch_s();
+    if (!B) return fl_fail(FL_ERR_ALLOC, "host allocation");
+    B->P = nprob;
+    (void)hipGetDevice(&B->device);
+    (void)hipDeviceGetAttribute(&B->cus, hipDeviceAttributeMultiprocessorCount, B->device);
+    B->voff.assign(nprob + 1, 0);
+    B->soff.assign(nprob + 1, 0);
+    std::vector<int> as(nprob + 1, 0), vs(nprob + 1, 0);
+    for (int i = 0; i < nprob; i++) {
+        const fl_geometry& g = geoms[i];
+        fl_plan pl = nullptr;
+        int rc = fl_plan_create(&pl, &g);
+        if (rc) {
+            b_free(B);
+            delete B;
+            return rc;
+        }
+        B->plans.push_back(pl);
+        BDesc d = {};
+        d.f = pl->fg;
+        d.a = pl->ag;
+        d.voff = B->voff[i];
+        d.soff = B->soff[i];
+        d.n = (int)pl->nvox;
+        B->voff[i + 1] = B->voff[i] + pl->nvox;
+        B->soff[i + 1] = B->soff[i] + pl->ndet * pl->nv;
+        as[i + 1] = as[i] + ((g.nx + 7) / 8) * ((g.ny + 7) / 8);
+        vs[i + 1] = vs[i] + (int)((pl->nvox + CH - 1) / CH);
+        d.adj0 = as[i];
+        d.vec0 = vs[i];
+        B->desc.push_back(d);
+    }
+    B->adj_total = as[nprob];
+    B->vec_total = vs[nprob];
+    hipError_t e = hipMalloc((void**)&B->d_desc, nprob * sizeof(BDesc));
+    if (e == hipSuccess) e = hipMalloc((void**)&B->d_fstart, (nprob + 1) * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void**)&B->d_astart, (nprob + 1) * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void**)&B->d_vstart, (nprob + 1) * sizeof(int));
+    if (e == hipSuccess) e = hipMemcpy(B->d_astart, as.data(), as.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(B->d_vstart, vs.data(), vs.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&B->ev_in, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&B->ev_out, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        b_free(B);
+        delete B;
+        return fl_fail(FL_ERR_ALLOC, "batch tables");
+    }
+    int rc = b_rebuild_fwd_table(B);
+    if (rc) {
+        b_free(B);
+        delete B;
+        return rc;
+    }
+    B->fwd_grid = 4 * B->cus;
+    B->adj_grid = 4 * B->cus;
+    B->vec_grid = 2 * B->cus;
+    *out = B;
+    return fl_ok();
+}
+
+extern "C" int fl_batch_destroy(fl_batch B)
+{
+    if (!B) return fl_ok();
+    b_free(B);
+    delete B;
+    return fl_ok();
+}
+
+extern "C" int fl_batch_layout(fl_batch B, long long* vol_offsets, long long* sino_offsets)
+{
+    if (!B) return fl_fail(FL_ERR_INVALID, "null batch");
+    for (int i = 0; i <= B->P; i++) {
+        if (vol_offsets) vol_offsets[i] = B->voff[i];
+        if (sino_offsets) sino_offsets[i] = B->soff[i];
+    }
+    return fl_ok();
+}
+
+static int b_forward(fl_batch_s* B, const float* x, float* y, bool gao, hipStream_t st)
+{
+    const unsigned g = (unsigned)std::min(B->fwd_grid, std::max(1, B->fwd_total));
+    if (gao)
+        hipLaunchKernelGGL(k_bfwd<true>, dim3(g), dim3(64), 0, st, x, y, B->d_desc, B->d_fstart, B->P, B->T,
+                           B->fwd_total);
+    else
+        hipLaunchKernelGGL(k_bfwd<false>, dim3(g), dim3(64), 0, st, x, y, B->d_desc, B->d_fstart, B->P, B->T,
+                           B->fwd_total);
+    BCHK(hipGetLastError());
+    return FL_OK;
+}
+
+static int b_adjoint(fl_batch_s* B, const float* y, float* x, bool matched, hipStream_t st)
+{
+    const unsigned g = (unsigned)std::min(B->adj_grid, std::max(1, B->adj_total));
+    if (matched)
+        hipLaunchKernelGGL(k_badj<true>, dim3(g), dim3(64), 0, st, y, x, B->d_desc, B->d_astart, B->P, B->adj_total);
+    else
+        hipLaunchKernelGGL(k_badj<false>, dim3(g), dim3(64), 0, st, y, x, B->d_desc, B->d_astart, B->P, B->adj_total);
+    BCHK(hipGetLastError());
+    return FL_OK;
+}
+
+extern "C" int fl_batch_forward(fl_batch B, const float* d_x, float* d_y, int method, void* stream)
+{
+    if (!B || !d_x || !d_y) return fl_fail(FL_ERR_INVALID, "bad batch forward arguments");
+    int rc = b_forward(B, d_x, d_y, method == 1, (hipStream_t)stream);
+    return rc ? rc : fl_ok();
+}
+
+extern "C" int fl_batch_adjoint(fl_batch B, const float* d_y, float* d_x, int kind, void* stream)
+{
+    if (!B || !d_x || !d_y || (kind != FL_ADJ_EXACT && kind != FL_ADJ_MATCHED))
+        return fl_fail(FL_ERR_INVALID, "bad batch adjoint arguments");
+    int rc = b_adjoint(B, d_y, d_x, kind == FL_ADJ_MATCHED, (hipStream_t)stream);
+    return rc ? rc : fl_ok();
+}
+
+static int b_ensure(fl_batch_s* B, int maxlog)
+{
+    const long long nv = B->voff[B->P], ns = B->soff[B->P];
+    if (!B->r) {
+        hipError_t e = hipMalloc((void**)&B->r, nv * 4);
+        if (e == hipSuccess) e = hipMalloc((void**)&B->p, nv * 4);
+        if (e == hipSuccess) e = hipMalloc((void**)&B->q, nv * 4);
+        if (e == hipSuccess) e = hipMalloc((void**)&B->sino, ns * 4);
+        if (e == hipSuccess) e = hipMalloc((void**)&B->part, (size_t)B->vec_total * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc((void**)&B->st, (size_t)B->P * sizeof(BState));
+        if (e != hipSuccess) return fl_fail(FL_ERR_ALLOC, "batch solver workspace");
+    }
+    if (maxlog > B->maxlog) {
+        b_drop_graph(B);   // the captured iteration holds the old log pointer
+        if (B->log) (void)hipFree(B->log);
+        B->log = nullptr;
+        if (hipMalloc((void**)&B->log, (size_t)B->P * maxlog * sizeof(double)) != hipSuccess)
+            return fl_fail(FL_ERR_ALLOC, "batch log");
+        B->maxlog = maxlog;
+    }
+    return FL_OK;
+}
+
+static int b_normal(fl_batch_s* B, const float* x, float* out, bool matched, hipStream_t st)
+{
+    int rc = b_forward(B, x, B->sino, false, st);
+    return rc ? rc : b_adjoint(B, B->sino, out, matched, st);
+}
+
+static int b_cg_body(fl_batch_s* B, float* u, float lam, bool matched, hipStream_t st)
+{
+    const unsigned g = (unsigned)std::min(B->vec_grid, B->vec_total);
+    int rc;
+    if ((rc = b_normal(B, B->p, B->q, matched, st))) return rc;
+    hipLaunchKernelGGL(k_bv_pq, dim3(g), dim3(256), 0, st, B->q, B->p, lam, B->part, B->st, B->d_desc, B->d_vstart,
+                       B->P, B->vec_total);
+    hipLaunchKernelGGL(k_bfin_alpha, dim3(B->P), dim3(256), 0, st, B->part, B->d_vstart, B->st);
+    hipLaunchKernelGGL(k_bv_update, dim3(g), dim3(256), 0, st, u, B->r, B->p, B->q, B->part, B->st, B->d_desc,
+                       B->d_vstart, B->P, B->vec_total);
+    hipLaunchKernelGGL(k_bfin_beta, dim3(B->P), dim3(256), 0, st, B->part, B->d_vstart, B->st, B->log, B->maxlog);
+    hipLaunchKernelGGL(k_bv_p, dim3(g), dim3(256), 0, st, B->p, B->r, B->st, B->d_desc, B->d_vstart, B->P,
+                       B->vec_total);
+    BCHK(hipGetLastError());
+    return FL_OK;
+}
+
+static int b_cg_run(fl_batch_s* B, const float* b, float* u, float lam, int iters, double tol, int flags,
+                    hipStream_t st)
+{
+    const bool matched = flags & FL_SOLVE_MATCHED;
+    const unsigned g = (unsigned)std::min(B->vec_grid, B->vec_total);
+    int rc;
+    if ((rc = b_normal(B, u, B->q, matched, st))) return rc;
+    hipLaunchKernelGGL(k_bv_init, dim3(g), dim3(256), 0, st, B->q, u, b, lam, B->r, B->p, B->part, B->d_desc,
+                       B->d_vstart, B->P, B->vec_total);
+    hipLaunchKernelGGL(k_bfin_gamma, dim3(B->P), dim3(256), 0, st, B->part, B->d_vstart, B->st, tol, B->log,
+                       B->maxlog);
+    BCHK(hipGetLastError());
+    if (flags & FL_SOLVE_GRAPH) {
+        const bool hit = B->gexec && B->gkey == (const void*)u && B->gkey_lam == lam && B->gkey_flags == flags;
+        if (!hit) {
+            b_drop_graph(B);
+            BCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            rc = b_cg_body(B, u, lam, matched, st);
+            hipGraph_t gr = nullptr;
+            hipError_t e = hipStreamEndCapture(st, &gr);
+            if (rc) {
+                if (gr) (void)hipGraphDestroy(gr);
+                return rc;
+            }
+            if (e != hipSuccess) return fl_hip_fail(e, "hipStreamEndCapture");
+            B->graph = gr;
+            BCHK(hipGraphInstantiate(&B->gexec, B->graph, nullptr, nullptr, 0));
+            B->gkey = u;
+            B->gkey_lam = lam;
+            B->gkey_flags = flags;
+        }
+        for (int k = 0; k < iters; k++) BCHK(hipGraphLaunch(B->gexec, st));
+    } else {
+        for (int k = 0; k < iters; k++)
+            if ((rc = b_cg_body(B, u, lam, matched, st))) return rc;
+    }
+    return FL_OK;
+}
+
+extern "C" int fl_batch_cgls(fl_batch B, const float* d_b, float* d_u, float lam, int iters, double tol, int flags,
+                             double* gamma_log, int* iters_done, void* stream)
+{
+    if (!B || !d_b || !d_u || iters < 0) return fl_fail(FL_ERR_INVALID, "bad batch CGLS arguments");
+    std::lock_guard<std::mutex> lk(B->mu);
+    hipStream_t st = B->stream;
+    int rc;
+    if ((rc = b_ensure(B, iters + 1))) return rc;
+    BCHK(hipEventRecord(B->ev_in, (hipStream_t)stream));
+    BCHK(hipStreamWaitEvent(st, B->ev_in, 0));
+    if ((rc = b_cg_run(B, d_b, d_u, lam, iters, tol, flags, st))) return rc;
+    if (gamma_log || iters_done) {
+        std::vector<BState> s(B->P);
+        BCHK(hipMemcpyAsync(s.data(), B->st, B->P * sizeof(BState), hipMemcpyDeviceToHost, st));
+        std::vector<double> lg((size_t)B->P * B->maxlog);
+        if (gamma_log) BCHK(hipMemcpyAsync(lg.data(), B->log, lg.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        BCHK(hipStreamSynchronize(st));
+        for (int p = 0; p < B->P; p++) {
+            if (iters_done) iters_done[p] = s[p].iter;
+            if (gamma_log)
+                for (int k = 0; k <= iters; k++)
+                    gamma_log[(size_t)p * (iters + 1) + k] = k <= s[p].iter ? lg[(size_t)p * B->maxlog + k] : s[p].gamma;
+        }
+    }
+    BCHK(hipEventRecord(B->ev_out, st));
+    BCHK(hipStreamWaitEvent((hipStream_t)stream, B->ev_out, 0));
+    return fl_ok();
+}
+
+template <int NF, int NT>
+static int b_sb_t(fl_batch_s* B, int kind, const float* f, float* u, float lam, float mu, int outer, int inner,
+                  double cg_tol, int flags, double* resid, hipStream_t st)
+{
+    const FlTaps T = fl_make_taps(kind);
+    const long long nv = B->voff[B->P];
+    const unsigned g = (unsigned)std::min(B->vec_grid, B->vec_total);
+    if (!B->V || B->nbands != NF * NF) {
+        for (void* q : {(void*)B->V, (void*)B->Dc, (void*)B->Atf, (void*)B->b})
+            if (q) (void)hipFree(q);
+        B->V = B->Dc = B->Atf = B->b = nullptr;
+        hipError_t e = hipMalloc((void**)&B->V, (size_t)NF * NF * nv * 4);
+        if (e == hipSuccess) e = hipMalloc((void**)&B->Dc, (size_t)NF * NF * nv * 4);
+        if (e == hipSuccess) e = hipMalloc((void**)&B->Atf, nv * 4);
+        if (e == hipSuccess) e = hipMalloc((void**)&B->b, nv * 4);
+        if (e != hipSuccess) return fl_fail(FL_ERR_ALLOC, "batch Bregman workspace");
+        B->nbands = NF * NF;
+    }
+    double* d_res = nullptr;
+    if (hipMalloc((void**)&d_res, (size_t)B->P * std::max(outer, 1) * sizeof(double)) != hipSuccess)
+        return fl_fail(FL_ERR_ALLOC, "batch residuals");
+    int rc = FL_OK;
+    const bool matched = flags & FL_SOLVE_MATCHED;
+    hipLaunchKernelGGL((k_bfwt2<NF, NT>), dim3(g), dim3(256), 0, st, u, B->V, T, B->d_desc, B->d_vstart, B->P,
+                       B->vec_total);
+    rc = b_adjoint(B, f, B->Atf, matched, st);
+    if (!rc) {
+        hipLaunchKernelGGL(k_bv_copy, dim3(1024), dim3(256), 0, st, B->Atf, B->b, nv);
+        for (int k = 0; k < outer && !rc; k++) {
+            if ((rc = b_cg_run(B, B->b, u, mu, inner, cg_tol, flags, st))) break;
+            hipLaunchKernelGGL((k_bbregman2<NF, NT>), dim3(g), dim3(256), 0, st, u, B->V, B->Dc, T, lam / mu,
+                               B->part, B->d_desc, B->d_vstart, B->P, B->vec_total);
+            hipLaunchKernelGGL(k_bfin_sum, dim3(B->P), dim3(256), 0, st, B->part, B->d_vstart, d_res,
+                               std::max(outer, 1), k);
+            hipLaunchKernelGGL((k_biwt2<NF, NT>), dim3(g), dim3(256), 0, st, B->Dc, B->b, mu, 1.0f, B->Atf, T,
+                               B->d_desc, B->d_vstart, B->P, B->vec_total);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) rc = fl_hip_fail(e, "batch Bregman step");
+        }
+    }
+    if (!rc && resid && outer > 0) {
+        hipError_t e = hipMemcpyAsync(resid, d_res, (size_t)B->P * outer * sizeof(double), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) rc = fl_hip_fail(e, "batch residual log");
+    }
+    if (!rc) {
+        hipError_t e = hipStreamSynchronize(st);
+        if (e != hipSuccess) rc = fl_hip_fail(e, "batch split Bregman");
+    }
+    (void)hipFree(d_res);
+    return rc;
+}
+
+extern "C" int fl_batch_split_bregman(fl_batch B, int kind, const float* d_f, float* d_u, float lam, float mu,
+                                      int outer, int inner, double cg_tol, int flags, double* resid, void* stream)
+{
+    if (!B || !d_f || !d_u || outer < 0 || inner < 0 || !(mu > 0))
+        return fl_fail(FL_ERR_INVALID, "bad batch split-Bregman arguments");
+    if (kind != FL_FRAME_HAAR && kind != FL_FRAME_LINEAR && kind != FL_FRAME_CUBIC)
+        return fl_fail(FL_ERR_INVALID, "bad framelet kind");
+    std::lock_guard<std::mutex> lk(B->mu);
+    hipStream_t st = B->stream;
+    int rc;
+    if ((rc = b_ensure(B, inner + 1))) return rc;
+    BCHK(hipEventRecord(B->ev_in, (hipStream_t)stream));
+    BCHK(hipStreamWaitEvent(st, B->ev_in, 0));
+    if (kind == FL_FRAME_HAAR) rc = b_sb_t<2, 3>(B, kind, d_f, d_u, lam, mu, outer, inner, cg_tol, flags, resid, st);
+    else if (kind == FL_FRAME_LINEAR)
+        rc = b_sb_t<3, 3>(B, kind, d_f, d_u, lam, mu, outer, inner, cg_tol, flags, resid, st);
+    else rc = b_sb_t<5, 5>(B, kind, d_f, d_u, lam, mu, outer, inner, cg_tol, flags, resid, st);
+    if (rc) return rc;
+    BCHK(hipEventRecord(B->ev_out, st));
+    BCHK(hipStreamWaitEvent((hipStream_t)stream, B->ev_out, 0));
+    return fl_ok();
+}
+
+// Time each candidate on the batch itself (events on a private stream)
+// and keep the fastest: ray-tile width T and persistent grid sizes.
+extern "C" int fl_batch_autotune(fl_batch B, int* fwd_tile, int* fwd_grid, int* adj_grid)
+{
+    if (!B) return fl_fail(FL_ERR_INVALID, "null batch");
+    std::lock_guard<std::mutex> lk(B->mu);
+    const long long nv = B->voff[B->P], ns = B->soff[B->P];
+    float *x = nullptr, *y = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = FL_OK;
+    if (hipMalloc((void**)&x, nv * 4) != hipSuccess || hipMalloc((void**)&y, ns * 4) != hipSuccess) {
+        rc = fl_fail(FL_ERR_ALLOC, "autotune buffers");
+    }
+    if (!rc && (hipStreamCreate(&st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+                hipEventCreate(&e1) != hipSuccess))
+        rc = fl_fail(FL_ERR_HIP, "autotune stream/events");
+    auto time_it = [&](auto&& launch) -> float {
+        launch();
+        (void)hipEventRecord(e0, st);
+        for (int k = 0; k < 3; k++) launch();
+        (void)hipEventRecord(e1, st);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        return ms;
+    };
+    if (!rc) {
+        (void)hipMemsetAsync(x, 0, nv * 4, st);
+        float best = 1e30f;
+        int bT = B->T, bG = B->fwd_grid;
+        for (int T : {64, 128, 256}) {
+            B->T = T;
+            if ((rc = b_rebuild_fwd_table(B))) break;
+            for (int m : {2, 4, 8, 16}) {
+                B->fwd_grid = m * B->cus;
+                float ms = time_it([&] { (void)b_forward(B, x, y, false, st); });
+                if (ms < best) { best = ms; bT = T; bG = B->fwd_grid; }
+            }
+        }
+        B->T = bT;
+        B->fwd_grid = bG;
+        if (!rc) rc = b_rebuild_fwd_table(B);
+        best = 1e30f;
+        int bA = B->adj_grid;
+        for (int m : {2, 4, 8, 16}) {
+            B->adj_grid = m * B->cus;
+            float ms = time_it([&] { (void)b_adjoint(B, y, x, false, st); });
+            if (ms < best) { best = ms; bA = B->adj_grid; }
+        }
+        B->adj_grid = bA;
+        b_drop_graph(B);   // a captured iteration holds the old launch shapes
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (x) (void)hipFree(x);
+    if (y) (void)hipFree(y);
+    if (rc) return rc;
+    if (fwd_tile) *fwd_tile = B->T;
+    if (fwd_grid) *fwd_grid = B->fwd_grid;
+    if (adj_grid) *adj_grid = B->adj_grid;
+    return fl_ok();
+}

@@ -19,17 +19,7 @@
 // read once and every band is written once (the transform is write-bound).
 // Synthesis gathers each band's neighbourhood and accumulates in FP32.
 #include "fl_common.h"
-
-struct FlTaps {
-    float D[5][5];   // decomposition filters, NT taps each
-    float R[5][5];   // reconstruction filters
-};
-
-__device__ __forceinline__ int wrapi(int v, int n)
-{
-    v %= n;
-    return v < 0 ? v + n : v;
-}
+#include "fl_framelet_dev.h"
 
 // ---------------------------------------------------------------- 3D ------
 
@@ -173,37 +163,16 @@ template <int NF, int NT, typename XT, typename CT>
 __global__ __launch_bounds__(256) void k_fwt2(const XT* __restrict__ u, CT* __restrict__ C, int nx, int ny,
                                              int step, FlTaps T)
 {
-    constexpr int CC = (NT - 1) / 2;
     const size_t npix = (size_t)nx * ny;
     const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (idx >= npix) return;
     const int x = (int)(idx % nx), y = (int)(idx / nx);
     float nb[NT][NT];
+    fl_gather2<NT>(u, nx, ny, x, y, step, nb);
+    float w[NF * NF];
+    fl_bands2<NF, NT>(T, nb, w);
 #pragma unroll
-    for (int ky = 0; ky < NT; ky++) {
-        const int yy = wrapi(y + (ky - CC) * step, ny);
-#pragma unroll
-        for (int kx = 0; kx < NT; kx++) nb[ky][kx] = fl_load(u, (size_t)yy * nx + wrapi(x + (kx - CC) * step, nx));
-    }
-    float tx[NF][NT];   // corr_x with D_i: [i][ky]
-#pragma unroll
-    for (int i = 0; i < NF; i++)
-#pragma unroll
-        for (int ky = 0; ky < NT; ky++) {
-            float s = 0.0f;
-#pragma unroll
-            for (int kx = 0; kx < NT; kx++) s = __builtin_fmaf(T.D[i][kx], nb[ky][kx], s);
-            tx[i][ky] = s;
-        }
-#pragma unroll
-    for (int i = 0; i < NF; i++)
-#pragma unroll
-        for (int j = 0; j < NF; j++) {
-            float s = 0.0f;
-#pragma unroll
-            for (int ky = 0; ky < NT; ky++) s = __builtin_fmaf(T.D[j][ky], tx[i][ky], s);
-            fl_store(C, (size_t)(i + NF * j) * npix + idx, s);
-        }
+    for (int b = 0; b < NF * NF; b++) fl_store(C, (size_t)b * npix + idx, w[b]);
 }
 
 template <int NF, int NT, typename CT, typename XT>
@@ -211,32 +180,11 @@ __global__ __launch_bounds__(256) void k_iwt2(const CT* __restrict__ C, XT* u, i
                                              int step, FlTaps T, float alpha, float beta,
                                              const float* base)
 {
-    constexpr int CC = (NT - 1) / 2;
     const size_t npix = (size_t)nx * ny;
     const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (idx >= npix) return;
     const int x = (int)(idx % nx), y = (int)(idx / nx);
-    int xs[NT], ys[NT];
-#pragma unroll
-    for (int k = 0; k < NT; k++) {
-        xs[k] = wrapi(x + (k - CC) * step, nx);
-        ys[k] = wrapi(y + (k - CC) * step, ny);
-    }
-    float acc = 0.0f;
-    for (int i = 0; i < NF; i++)
-        for (int j = 0; j < NF; j++) {
-            const CT* __restrict__ cb = C + (size_t)(i + NF * j) * npix;
-            float sx = 0.0f;   // corr_y[R_j] innermost, then corr_x[R_i]
-#pragma unroll
-            for (int kx = 0; kx < NT; kx++) {
-                float sy = 0.0f;
-#pragma unroll
-                for (int ky = 0; ky < NT; ky++) sy = __builtin_fmaf(T.R[j][ky], fl_load(cb, (size_t)ys[ky] * nx + xs[kx]), sy);
-                sx = __builtin_fmaf(T.R[i][kx], sy, sx);
-            }
-            acc += sx;
-        }
-    float v = alpha * acc;
+    float v = alpha * fl_synth2<NF, NT>(C, npix, nx, ny, x, y, step, T);
     if (base) v = __builtin_fmaf(beta, base[idx], v);
     fl_store(u, idx, v);
 }
@@ -253,12 +201,6 @@ __global__ __launch_bounds__(256) void k_iwt2(const CT* __restrict__ C, XT* u, i
 // Smoothed l1 (gradient of lam * sum psi(Wu), psi(t) = sqrt(t^2 + eps)):
 //   g_b = lam * psi'(Wu_b) = lam * Wu_b / sqrt(Wu_b^2 + eps), low-pass band 0
 //   unless with_lowpass; loss partials of lam * sum psi.
-
-__device__ __forceinline__ float shrinkf(float t, float tau)
-{
-    float m = fabsf(t) - tau;
-    return m > 0.0f ? copysignf(m, t) : 0.0f;
-}
 
 template <int NB>
 __device__ __forceinline__ void block_reduce_store(double v, double* __restrict__ part)
@@ -310,46 +252,12 @@ __global__ __launch_bounds__(256) void k_bregman2(const float* __restrict__ u, C
                                                  CT* __restrict__ Dc, int nx, int ny, int step, FlTaps T,
                                                  float tau, double* __restrict__ part)
 {
-    constexpr int CC = (NT - 1) / 2;
     const size_t npix = (size_t)nx * ny;
     const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
     double r2 = 0.0;
     if (idx < npix) {
         const int x = (int)(idx % nx), y = (int)(idx / nx);
-        float nb[NT][NT];
-#pragma unroll
-        for (int ky = 0; ky < NT; ky++) {
-            const int yy = wrapi(y + (ky - CC) * step, ny);
-#pragma unroll
-            for (int kx = 0; kx < NT; kx++) nb[ky][kx] = u[(size_t)yy * nx + wrapi(x + (kx - CC) * step, nx)];
-        }
-#pragma unroll
-        for (int i = 0; i < NF; i++) {
-            float tx[NT];
-#pragma unroll
-            for (int ky = 0; ky < NT; ky++) {
-                float s = 0.0f;
-#pragma unroll
-                for (int kx = 0; kx < NT; kx++) s = __builtin_fmaf(T.D[i][kx], nb[ky][kx], s);
-                tx[ky] = s;
-            }
-#pragma unroll
-            for (int j = 0; j < NF; j++) {
-                float w = 0.0f;
-#pragma unroll
-                for (int ky = 0; ky < NT; ky++) w = __builtin_fmaf(T.D[j][ky], tx[ky], w);
-                const int b = i + NF * j;
-                const size_t o = (size_t)b * npix + idx;
-                const float v = fl_load(V, o);
-                const float t = v + w;
-                const float a = (b == 0) ? t : shrinkf(t, tau);
-                const float e = w - a;
-                const float vn = v + e;
-                r2 += (double)e * (double)e;
-                fl_store(V, o, vn);
-                fl_store(Dc, o, a - vn);
-            }
-        }
+        r2 = fl_bregman2_pixel<NF, NT>(u, V, Dc, npix, idx, nx, ny, x, y, step, T, tau);
     }
     block_reduce_store<NF * NF>(r2, part);
 }
@@ -434,7 +342,7 @@ __global__ __launch_bounds__(256) void k_l1grad2(const float* __restrict__ u, CT
 
 #include "fl_framelet.h"
 
-static FlTaps make_taps(int kind)
+FlTaps fl_make_taps(int kind)
 {
     FlTaps T = {};
     const double s2 = 1.4142135623730951, s6 = 2.449489742783178;
@@ -479,7 +387,7 @@ static unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
 template <typename XT, typename CT>
 static hipError_t fwt_t(const FlFrameDesc& f, const void* u, void* C, int level, hipStream_t st)
 {
-    const FlTaps T = make_taps(f.kind);
+    const FlTaps T = fl_make_taps(f.kind);
     const int step = 1 << (level - 1);
     const size_t n = (size_t)f.nx * f.ny * (f.dim == 3 ? f.nz : 1);
     if (f.dim == 3)
@@ -495,7 +403,7 @@ template <typename CT, typename XT>
 static hipError_t iwt_t(const FlFrameDesc& f, const void* C, void* u, int level, float alpha, float beta,
                         const float* base, hipStream_t st)
 {
-    const FlTaps T = make_taps(f.kind);
+    const FlTaps T = fl_make_taps(f.kind);
     const int step = 1 << (level - 1);
     const size_t n = (size_t)f.nx * f.ny * (f.dim == 3 ? f.nz : 1);
     if (f.dim == 3)
@@ -534,7 +442,7 @@ template <typename CT>
 static hipError_t bregman_t(const FlFrameDesc& f, const float* u, void* V, void* D, float tau, double* part,
                             hipStream_t st)
 {
-    const FlTaps T = make_taps(f.kind);
+    const FlTaps T = fl_make_taps(f.kind);
     const size_t n = (size_t)f.nx * f.ny * (f.dim == 3 ? f.nz : 1);
     if (f.dim == 3)
         FL_FRAME_SWITCH(f.kind, hipLaunchKernelGGL((k_bregman3<NF, NT, CT>), dim3(nblk(n)), dim3(256), 0, st, u,
@@ -557,7 +465,7 @@ hipError_t fl_launch_bregman(const FlFrameDesc& f, const float* u, void* V, void
 hipError_t fl_launch_l1grad(const FlFrameDesc& f, const float* u, float* G, float lam, float eps, int with_lp,
                             double* part, hipStream_t st)
 {
-    const FlTaps T = make_taps(f.kind);
+    const FlTaps T = fl_make_taps(f.kind);
     const size_t n = (size_t)f.nx * f.ny * (f.dim == 3 ? f.nz : 1);
     if (f.dim == 3)
         FL_FRAME_SWITCH(f.kind, hipLaunchKernelGGL((k_l1grad3<NF, NT, float>), dim3(nblk(n)), dim3(256), 0, st, u,

@@ -22,29 +22,9 @@
 #include "fl_geom.h"
 #include "fl_siddon.h"
 #include "fl_chord.h"
+#include "fl_fan.h"
 
-namespace {
 
-struct AxisIv {
-    float in, out;  // entry / exit parameters of the voxel slab along one axis
-};
-
-__device__ __forceinline__ AxisIv slab(const SidAxis& a, int cell)
-{
-    AxisIv r;
-    if (!a.on) {
-        r.in = -FL_INF;
-        r.out = FL_INF;
-        return r;
-    }
-    float lo = ((float)(cell - a.nh) * a.s - a.o) / a.d;
-    float hi = ((float)(cell - a.nh + 1) * a.s - a.o) / a.d;
-    r.in = a.st > 0 ? lo : hi;
-    r.out = a.st > 0 ? hi : lo;
-    return r;
-}
-
-}  // namespace
 
 // Per (ray, voxel) pair of one view: the Siddon segment of the ray inside
 // the voxel, credited when the forward's midpoint rule maps it there.
@@ -199,14 +179,14 @@ __global__ __launch_bounds__(256) void k_adj_matched_cone(const YT* __restrict__
                 const float dz = (f.zdet[ib] + sz) - z1;
                 SidAxis az;
                 sid_axis_init(az, z1, dz, f.dz0, f.nz);
-                const AxisIv sv = slab(az, iz);
+                const FlAxisIv sv = fl_slab(az, iz);
                 for (int ia = alo; ia <= ahi; ia++) {
                     const float yd = f.ydet[ia];
                     const float dx = (c * f.OD - s * yd) - x1, dy = (s * f.OD + c * yd) - y1;
                     SidAxis ax, ay;
                     sid_axis_init(ax, x1, dx, 1.0f, f.nx);
                     sid_axis_init(ay, y1, dy, 1.0f, f.ny);
-                    const AxisIv sx = slab(ax, ix), sy = slab(ay, iy);
+                    const FlAxisIv sx = fl_slab(ax, ix), sy = fl_slab(ay, iy);
                     const float L = sqrtf(dx * dx + dy * dy + dz * dz);
                     const float Si = matched_coef(sx.in, sx.out, sy.in, sy.out, sv.in, sv.out, ax, ay, az, ix, iy,
                                                   iz, dz_one, L);
@@ -226,42 +206,7 @@ __global__ __launch_bounds__(256) void k_adj_matched_fan(const YT* __restrict__ 
     const int iy = blockIdx.y * 16 + threadIdx.y;
     const int it = blockIdx.z;
     if (ix >= f.nx || iy >= f.ny) return;
-    const int nd = f.na;
-    const float SD = f.SO + f.OD;
-    const float xc = (float)ix + 0.5f - (float)(f.nx / 2);
-    const float yc = (float)iy + 0.5f - (float)(f.ny / 2);
-    const float ya0 = f.ydet[0];
-    float acc = 0.0f;
-    const int nvf = g.Nv[it];
-    for (int j = 0; j < nvf; j++) {
-        const int iv = g.id_Y[it * g.tmp_size + j];
-        const YT* __restrict__ y = Y + (size_t)iv * nd;
-        const float c = f.cs[iv], s = f.sn[iv];
-        const float xr = c * xc + s * yc, yr = -s * xc + c * yc;
-        const float mag = SD / (xr + f.SO);
-        int lo = (int)floorf(((yr - 1.0f) * mag - ya0) / g.dy_det) - 1;
-        int hi = (int)ceilf(((yr + 1.0f) * mag - ya0) / g.dy_det) + 1;
-        if (lo < 0) lo = 0;
-        if (hi > nd - 1) hi = nd - 1;
-        const float x1 = c * (-f.SO), y1 = s * (-f.SO);
-        for (int id = lo; id <= hi; id++) {
-            const float yd = f.ydet[id];
-            const float x2 = c * f.OD - s * yd;
-            const float y2 = s * f.OD + c * yd;
-            const float dx = x2 - x1, dy = y2 - y1;
-            SidAxis ax, ay;
-            sid_axis_init(ax, x1, dx, 1.0f, f.nx);
-            sid_axis_init(ay, y1, dy, 1.0f, f.ny);
-            const AxisIv sx = slab(ax, ix), sy = slab(ay, iy);
-            const float ain = fmaxf(sx.in, sy.in), aout = fminf(sx.out, sy.out);
-            if (!(aout > ain)) continue;
-            const float m = (ain + aout) * 0.5f;
-            if (sid_cell_xy(ax, m) != ix || sid_cell_xy(ay, m) != iy) continue;
-            const float L = sqrtf(dx * dx + dy * dy);
-            acc = acc + ((aout - ain) * L) * fl_load(y, (size_t)id);
-        }
-    }
-    fl_store(X, (size_t)it * f.nvox + (size_t)iy * f.nx + ix, acc * f.scale);
+    fl_store(X, (size_t)it * f.nvox + (size_t)iy * f.nx + ix, fl_fan_matched_pixel(Y, f, g, it, ix, iy));
 }
 
 template <typename YT, typename XT>

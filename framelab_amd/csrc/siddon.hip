@@ -13,6 +13,7 @@
 #include "fl_geom.h"
 #include "fl_siddon.h"
 #include "fl_chord.h"
+#include "fl_fan.h"
 
 namespace {
 
@@ -98,19 +99,7 @@ __global__ __launch_bounds__(64) void k_fwd_fan(const VT* __restrict__ X, YT* __
     const int id = (int)(b % tiles) * 64 + (int)threadIdx.x;
     if (slot >= g.nslots || id >= g.na) return;
     const int iv = g.slot_view[slot];
-    const VT* __restrict__ x = X + (size_t)g.view_frame[iv] * g.nvox;
-    RayGeo r = fan_ray(g, iv, id);
-    SidAxis ax, ay, az;
-    sid_axis_init(ax, r.x1, r.dx, 1.0f, g.nx);
-    sid_axis_init(ay, r.y1, r.dy, 1.0f, g.ny);
-    sid_axis_init(az, 0.0f, 0.0f, 1.0f, 2);
-    const float L = r.L;
-    float acc = 0.0f;
-    sid_walk<false>(ax, ay, az, L, true, [&](int J, float dal) {
-        float Si = dal * L;
-        acc = acc + Si * fl_load(x, (size_t)J);
-    });
-    fl_store(Y, (size_t)slot * g.na + id, acc * g.scale);
+    fl_store(Y, (size_t)slot * g.na + id, fl_fan_ray_sum(X + (size_t)g.view_frame[iv] * g.nvox, g, iv, id));
 }
 
 // Traversal log of selected rays (parity tests): ray k = (view, ia, ib)
@@ -198,10 +187,6 @@ hipError_t fl_launch_trace(const FwdGeom& g, int cone, int nrays, const int* ray
 }
 
 // ---- Gao column-walk fan forward (class default, version 1) -------------
-// matlab/mex/src/Ax_fan_mf_cpu_new.cpp:28-119: walk unit columns of the
-// major axis; each column's crossing interval [c1, c2] covers one or two
-// cells whose lengths are the exact split of sqrt(1+slope^2).  The y-major
-// branch keeps the reference's `c2 == ny` test (cpu :111).
 template <typename VT, typename YT>
 __global__ __launch_bounds__(64) void k_fwd_fan_gao(const VT* __restrict__ X, YT* __restrict__ Y, FwdGeom g)
 {
@@ -211,76 +196,7 @@ __global__ __launch_bounds__(64) void k_fwd_fan_gao(const VT* __restrict__ X, YT
     const int id = (int)(b % tiles) * 64 + (int)threadIdx.x;
     if (slot >= g.nslots || id >= g.na) return;
     const int iv = g.slot_view[slot];
-    const VT* __restrict__ x = X + (size_t)g.view_frame[iv] * g.nvox;
-    const int nx = g.nx, ny = g.ny, nx2 = nx / 2, ny2 = ny / 2;
-    const float c = g.cs[iv], s = g.sn[iv];
-    const float x1 = c * (-g.SO), y1 = s * (-g.SO);
-    const float yd = g.ydet[id];
-    const float x2 = c * g.OD - s * yd;
-    const float y2 = s * g.OD + c * yd;
-    float acc = 0.0f;
-    if (fl_absf(x1 - x2) > fl_absf(y1 - y2)) {
-        const float slope = (y2 - y1) / (x2 - x1);
-        for (int ix = 0; ix < nx; ix++) {
-            const float u1 = (float)(ix - nx2), u2 = u1 + 1;
-            float v1, v2;
-            if (slope >= 0) { v1 = y1 + slope * (u1 - x1) + (float)ny2; v2 = y1 + slope * (u2 - x1) + (float)ny2; }
-            else            { v1 = y1 + slope * (u2 - x1) + (float)ny2; v2 = y1 + slope * (u1 - x1) + (float)ny2; }
-            const int c1 = (int)floorf(v1), c2 = (int)floorf(v2);
-            if (c2 == c1) {
-                if (c1 >= 0 && c1 <= ny - 1) {
-                    float d = v2 - v1, l = sqrtf(d * d + 1);
-                    acc = acc + l * fl_load(x, (size_t)c1 * nx + ix);
-                }
-            } else if (c2 > 0 && c2 < ny) {
-                // c2 - c1 == 1 whenever |slope| < 1; a float slope that
-                // rounds to 1 can give c1 == c2 - 2 = -1, which is skipped
-                float d = v2 - v1, l = sqrtf(d * d + 1);
-                if (c1 >= 0) acc = acc + (((float)c2 - v1) / d) * l * fl_load(x, (size_t)c1 * nx + ix);
-                acc = acc + ((v2 - (float)c2) / d) * l * fl_load(x, (size_t)c2 * nx + ix);
-            } else {
-                if (c2 == 0) {
-                    float d = v2 - v1, l = sqrtf(d * d + 1);
-                    acc = acc + ((v2 - (float)c2) / d) * l * fl_load(x, (size_t)c2 * nx + ix);
-                }
-                if (c2 == ny) {
-                    float d = v2 - v1, l = sqrtf(d * d + 1);
-                    acc = acc + (((float)c2 - v1) / d) * l * fl_load(x, (size_t)c1 * nx + ix);
-                }
-            }
-        }
-    } else {
-        const float slope = (x2 - x1) / (y2 - y1);
-        for (int iy = 0; iy < ny; iy++) {
-            const float u1 = (float)(iy - ny2), u2 = u1 + 1;
-            float v1, v2;
-            if (slope >= 0) { v1 = x1 + slope * (u1 - y1) + (float)nx2; v2 = x1 + slope * (u2 - y1) + (float)nx2; }
-            else            { v1 = x1 + slope * (u2 - y1) + (float)nx2; v2 = x1 + slope * (u1 - y1) + (float)nx2; }
-            const int c1 = (int)floorf(v1), c2 = (int)floorf(v2);
-            if (c2 == c1) {
-                if (c1 >= 0 && c1 <= nx - 1) {
-                    float d = v2 - v1, l = sqrtf(d * d + 1);
-                    acc = acc + l * fl_load(x, (size_t)iy * nx + c1);
-                }
-            } else if (c2 > 0 && c2 < nx) {
-                float d = v2 - v1, l = sqrtf(d * d + 1);
-                if (c1 >= 0) acc = acc + (((float)c2 - v1) / d) * l * fl_load(x, (size_t)iy * nx + c1);
-                acc = acc + ((v2 - (float)c2) / d) * l * fl_load(x, (size_t)iy * nx + c2);
-            } else {
-                if (c2 == 0) {
-                    float d = v2 - v1, l = sqrtf(d * d + 1);
-                    acc = acc + ((v2 - (float)c2) / d) * l * fl_load(x, (size_t)iy * nx + c2);
-                }
-                // the reference tests ny here; with nx != ny only cells
-                // c1 = ny-1 < nx are reachable, so the index stays in range
-                if (c2 == ny && c1 >= 0 && c1 < nx) {
-                    float d = v2 - v1, l = sqrtf(d * d + 1);
-                    acc = acc + (((float)c2 - v1) / d) * l * fl_load(x, (size_t)iy * nx + c1);
-                }
-            }
-        }
-    }
-    fl_store(Y, (size_t)slot * g.na + id, acc * g.scale);
+    fl_store(Y, (size_t)slot * g.na + id, fl_fan_gao_sum(X + (size_t)g.view_frame[iv] * g.nvox, g, iv, id));
 }
 
 template <typename VT, typename YT>

@@ -194,6 +194,34 @@ int fl_grad_framelet(int dim, int kind, int nx, int ny, int nz, const float *d_x
                      float eps, int with_lowpass, float *d_g, float *d_work, double *loss,
                      void *stream);
 
+/* ------------------------------------------------------------- batches
+ * Many independent 2D fan-beam problems of uneven sizes (one frame each),
+ * each operator one persistent launch over all of them.  Volumes and
+ * sinograms are packed back to back in problem order (fl_batch_layout gives
+ * the P+1 offsets, in floats).  Per-problem results are identical to the
+ * single-plan calls (same per-ray / per-pixel arithmetic); CGLS and split
+ * Bregman keep per-problem FP64 scalars and stopping tests.  No reference
+ * counterpart: the reference runs one problem per call. */
+typedef struct fl_batch_s *fl_batch;
+
+int fl_batch_create(fl_batch *batch, int nprob, const fl_geometry *geoms);
+int fl_batch_destroy(fl_batch batch);
+int fl_batch_layout(fl_batch batch, long long *vol_offsets, long long *sino_offsets);
+/* method 0: Siddon, 1: Gao column walk */
+int fl_batch_forward(fl_batch batch, const float *d_x, float *d_y, int method, void *stream);
+int fl_batch_adjoint(fl_batch batch, const float *d_y, float *d_x, int kind, void *stream);
+/* gamma_log: nprob*(iters+1) doubles, iters_done: nprob ints (optional) */
+int fl_batch_cgls(fl_batch batch, const float *d_b, float *d_u, float lam, int iters, double tol,
+                  int flags, double *gamma_log, int *iters_done, void *stream);
+/* resid_log: nprob*outer doubles (optional) */
+int fl_batch_split_bregman(fl_batch batch, int frame_kind, const float *d_f, float *d_u, float lam,
+                           float mu, int outer, int inner, double cg_tol, int flags,
+                           double *resid_log, void *stream);
+/* Times the candidate ray-tile widths and persistent grid sizes on this
+ * batch and keeps the fastest (outputs optional).  Must not overlap other
+ * calls on the batch. */
+int fl_batch_autotune(fl_batch batch, int *fwd_tile, int *fwd_grid, int *adj_grid);
+
 /* Traversal log of selected rays (testing): rays = (view, ia, ib) triples
  * (host), cap entries per ray; J/S/cnt host outputs of nrays*cap / nrays. */
 int fl_siddon_trace(fl_plan plan, int nrays, const int *rays, int cap, int *J, float *S, int *cnt);

@@ -51,6 +51,21 @@ def test_error_strings_without_gpu():
     assert b"non-positive" in L.fl_last_error_message() or b"even" in L.fl_last_error_message()
 
 
+def test_batch_rejects_non_fan_without_gpu():
+    """Batch geometry checks run before any device call."""
+    from framelab_amd import _lib
+    L = _lib.lib()
+    gs = (_lib.FlGeometry * 2)()
+    for g in gs:
+        g.nx = g.ny = 16
+        g.nt, g.nv, g.na, g.nb = 1, 8, 16, 1
+    gs[1].nz = 16          # a cone geometry
+    h = C.c_void_p()
+    assert L.fl_batch_create(C.byref(h), 2, gs) == _lib.FL_ERR_INVALID and not h.value
+    assert b"problem 1" in L.fl_last_error_message()
+    assert L.fl_batch_create(C.byref(h), 0, gs) == _lib.FL_ERR_INVALID
+
+
 def test_fan_para_matches_setPara_formulas():
     """FanBeamScanner/setPara.m:5-30 with the reference defaults SO=OD=5, Ly=6."""
     p = G.fan_para(256, 180, N=(128, 128), L=(2.0, 2.0))
