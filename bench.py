@@ -81,6 +81,24 @@ def cpu_baseline(p, X, views=3):
                       f"(t_A={t1 - t0:.2f}s, t_At={t2 - t1:.2f}s); host: {model}, {cores} logical CPUs"}
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, calibrated against a 1 GiB copy),
+    or None when the kernel has no entry."""
+    path = os.path.join(ROOT, "profiles", "cone256_hbm_traffic.tsv")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("#"):
+                    continue
+                name, rd, wr = line.rstrip("\n").split("\t")
+                if name == kernel:
+                    return float(rd) + float(wr)
+    except OSError:
+        pass
+    return None
+
+
 def time_cgls(P, b, u0, iters, **kw):
     """Seconds per CGLS iteration (the init's normal-operator pass excluded)."""
     import torch
@@ -98,6 +116,128 @@ def time_cgls(P, b, u0, iters, **kw):
     return (t - t_init) / iters, u
 
 
+def batch_problems(count=64, seed=5):
+    """64 independent fan problems of seeded uneven sizes: image side from
+    {128,192,256,384,512}, views U[90,720], bins U[256,1024], 6-15 ellipses,
+    source-to-centre 4, source-to-detector 8, full fan."""
+    import numpy as np
+    from framelab_amd import geometry as G
+    from framelab_amd.phantoms import ellipses_2d
+    rng = np.random.default_rng(seed)
+    paras, xs = [], []
+    for i in range(count):
+        N = int(rng.choice([128, 192, 256, 384, 512]))
+        nv = int(rng.integers(90, 721))
+        nd = int(rng.integers(256, 1025))
+        paras.append(G.fan_para(nd, nv, N=(N, N), SO=4.0, OD=4.0, Ly=G.full_fan_width()))
+        xs.append(ellipses_2d(N, int(rng.integers(6, 16)), seed=seed * 1000 + i).astype(np.float32))
+    return paras, xs
+
+
+def bench_batch(dev, cgls_iters=30, outer=5, inner=3):
+    """Batched leg: aggregate proj+backproj GUPS of one persistent launch per
+    operator against a loop of single-problem calls; batched CGLS (graph
+    replay, FP64 per-problem reductions) iterations/s against the loop, with
+    each problem's relative L2 difference to its single-plan reconstruction
+    and to the phantom; split Bregman with the 9-band linear framelet."""
+    import numpy as np
+    import torch
+    from framelab_amd.batch import ProjectorBatch
+    from framelab_amd.projector import Projector
+    from framelab_amd.solvers import cgls
+
+    paras, xs = batch_problems()
+    B = ProjectorBatch(paras)
+    Ps = [Projector(p) for p in paras]
+    x = B.pack([torch.from_numpy(v) for v in xs]).to(dev)
+    y = torch.empty(B.n_sino, device=dev)
+    xb = torch.empty(B.n_vol, device=dev)
+    upd = float(sum(p.nv * p.nx * p.ny for p in paras))
+
+    def t_batched(reps=3):
+        B.forward(x, out=y)
+        B.adjoint(y, out=xb)
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        e[0].record()
+        for _ in range(reps):
+            B.forward(x, out=y)
+            B.adjoint(y, out=xb)
+        e[1].record()
+        torch.cuda.synchronize()
+        return e[0].elapsed_time(e[1]) / reps
+
+    t_default = t_batched()
+    tuned = B.autotune()
+    t_tuned = t_batched()
+    xi = [x[B.vol_offsets[i]:B.vol_offsets[i + 1]].view(B.vol_shapes[i]) for i in range(len(paras))]
+    yi = [torch.empty(B.sino_shapes[i], device=dev) for i in range(len(paras))]
+    for i, P in enumerate(Ps):
+        P.adjoint(P.forward(xi[i], out=yi[i]))
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    e[0].record()
+    for i, P in enumerate(Ps):
+        P.adjoint(P.forward(xi[i], out=yi[i]))
+    e[1].record()
+    torch.cuda.synchronize()
+    t_loop = e[0].elapsed_time(e[1])
+
+    # CGLS on the noiseless normal equations, all problems in lock step
+    f = B.forward(x)
+    b = B.adjoint(f)
+    u0 = torch.zeros_like(b)
+    B.cgls(b, u0, iters=1, graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    B.cgls(b, u0, iters=0, graph=True)
+    torch.cuda.synchronize()
+    t_init = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    u, glog, done = B.cgls(b, u0, iters=cgls_iters, graph=True)
+    torch.cuda.synchronize()
+    s_batch = (time.perf_counter() - t0 - t_init) / cgls_iters
+    us = B.unpack(u)
+    bs = B.unpack(b)
+    rel_single, rel_phantom = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    singles = [cgls(P, bs[i], torch.zeros_like(bs[i]), 0.0, cgls_iters, log=False, graph=True)[0]
+               for i, P in enumerate(Ps)]
+    torch.cuda.synchronize()
+    s_loop = (time.perf_counter() - t0) / cgls_iters
+    for i in range(len(paras)):
+        rel_single.append(float((us[i] - singles[i]).norm() / singles[i].norm()))
+        rel_phantom.append(float((us[i] - xi[i]).norm() / xi[i].norm()))
+
+    # split Bregman, 9-band linear framelet
+    B.split_bregman(f, u0, frame="linear", lam=1e-3, mu=1.0, outer=1, inner=inner, graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ub, res = B.split_bregman(f, u0, frame="linear", lam=1e-3, mu=1.0, outer=outer, inner=inner, graph=True)
+    torch.cuda.synchronize()
+    t_sb = time.perf_counter() - t0
+    ubs = B.unpack(ub)
+    rel_sb = [float((ubs[i] - xi[i]).norm() / xi[i].norm()) for i in range(len(paras))]
+    sides = [p.nx for p in paras]
+    return {
+        "workload": "64 fan problems, side in {128..512}, views U[90,720], bins U[256,1024], SO=4 SD=8",
+        "problems": len(paras), "sides_hist": {str(k): sides.count(k) for k in sorted(set(sides))},
+        "updates_per_operator": upd,
+        "proj_backproj_gups_batched": 2 * upd / (t_tuned * 1e-3) / 1e9,
+        "proj_backproj_gups_batched_untuned": 2 * upd / (t_default * 1e-3) / 1e9,
+        "proj_backproj_gups_single_loop": 2 * upd / (t_loop * 1e-3) / 1e9,
+        "autotuned": tuned,
+        "cgls_iters": cgls_iters,
+        "cgls_iters_per_s_batched": 1.0 / s_batch,
+        "cgls_iters_per_s_single_loop": 1.0 / s_loop,
+        "cgls_rel_l2_vs_single_max": max(rel_single),
+        "cgls_rel_l2_vs_phantom_median": float(np.median(rel_phantom)),
+        "cgls_rel_l2_vs_phantom_max": max(rel_phantom),
+        "split_bregman": {"outer": outer, "inner": inner, "frame": "linear (9 bands)", "lam": 1e-3, "mu": 1.0,
+                          "seconds": t_sb, "rel_l2_vs_phantom_median": float(np.median(rel_sb)),
+                          "final_resid_max": float(res[:, -1].max())},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +247,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-cgls", action="store_true", help="skip the CGLS leg")
     ap.add_argument("--cpu-views", type=int, default=3)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 2D leg")
+    ap.add_argument("--profile", action="store_true",
+                    help="counter run: one forward, one exact and one matched adjoint at 256^3, then a "
+                         "plain 1 GiB device copy (the calibration kernel for HBM byte counters); no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); multi-GPU sharding is "
@@ -142,6 +286,17 @@ def main():
         if evs is not None:
             evs[2].record()
 
+    if args.profile:
+        P.forward(x, out=y)
+        P.adjoint(y, out=xb, kind=ADJ_EXACT)
+        P.adjoint(y, out=xb, kind=ADJ_MATCHED)
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("bench: profile run done (forward, exact adjoint, matched adjoint, 1 GiB copy)")
+        return
+
     log("bench: warmup")
     for _ in range(args.warmup):
         step()
@@ -164,8 +319,9 @@ def main():
     dom_ms, dom_name = (mAt, "k_adj_cone_fp<float,float>") if mAt >= mA else (mA, "k_fwd_cone<float,float,64,4>")
     achieved = B_op / (dom_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom_name,
-            "algorithmic_bytes_per_launch": B_op, "kernel_ms": dom_ms}
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom_name), "kernel": dom_name,
+            "algorithmic_bytes_per_launch": B_op, "kernel_ms": dom_ms,
+            "traffic_source": "profiles/cone256_hbm_traffic_pmc.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"}
 
     # matched adjoint and low-precision storage variants (reported beside)
     extra = {"t_A_ms": mA, "t_At_exact_ms": mAt}
@@ -225,6 +381,14 @@ def main():
                      "graph-replayed",
         }
         log(f"bench: cgls fp32 {1 / s32:.2f} it/s, mixed {1 / smx:.2f} it/s, rel {rel:.2e}")
+
+    if not args.no_batch:
+        log("bench: batch")
+        result["batch"] = bench_batch(dev)
+        log(f"bench: batch {result['batch']['proj_backproj_gups_batched']:.1f} GUPS "
+            f"(loop {result['batch']['proj_backproj_gups_single_loop']:.1f}), cgls "
+            f"{result['batch']['cgls_iters_per_s_batched']:.2f} it/s (loop "
+            f"{result['batch']['cgls_iters_per_s_single_loop']:.2f})")
 
     if not args.no_cpu:
         log("bench: cpu baseline")

@@ -327,6 +327,7 @@ struct fl_batch_s {
     int fwd_grid = 0, adj_grid = 0, vec_grid = 0;
     // solver workspace
     float *r = nullptr, *p = nullptr, *q = nullptr, *sino = nullptr;
+    float* u = nullptr;      // the iterate while a solve runs (graphs capture this address)
     double* part = nullptr;
     BState* st = nullptr;
     double* log = nullptr;
@@ -335,7 +336,6 @@ struct fl_batch_s {
     int nbands = 0;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
-    const void* gkey = nullptr;
     float gkey_lam = 0.0f;
     int gkey_flags = -1;
     // solvers run on a private stream ordered after / before the caller's
@@ -350,7 +350,7 @@ static void b_drop_graph(fl_batch_s* B)
     if (B->graph) (void)hipGraphDestroy(B->graph);
     B->gexec = nullptr;
     B->graph = nullptr;
-    B->gkey = nullptr;
+    B->gkey_flags = -1;
 }
 
 #define BCHK(expr)                                            \
@@ -376,7 +376,7 @@ static int b_rebuild_fwd_table(fl_batch_s* B)
 static void b_free(fl_batch_s* B)
 {
     for (fl_plan pl : B->plans) fl_plan_destroy(pl);
-    void* ptrs[] = {B->d_desc, B->d_fstart, B->d_astart, B->d_vstart, B->r, B->p, B->q, B->sino, B->part,
+    void* ptrs[] = {B->d_desc, B->d_fstart, B->d_astart, B->d_vstart, B->r, B->p, B->q, B->sino, B->u, B->part,
                     B->st, B->log, B->V, B->Dc, B->Atf, B->b};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -527,6 +527,7 @@ static int b_ensure(fl_batch_s* B, int maxlog)
         if (e == hipSuccess) e = hipMalloc((void**)&B->p, nv * 4);
         if (e == hipSuccess) e = hipMalloc((void**)&B->q, nv * 4);
         if (e == hipSuccess) e = hipMalloc((void**)&B->sino, ns * 4);
+        if (e == hipSuccess) e = hipMalloc((void**)&B->u, nv * 4);
         if (e == hipSuccess) e = hipMalloc((void**)&B->part, (size_t)B->vec_total * sizeof(double));
         if (e == hipSuccess) e = hipMalloc((void**)&B->st, (size_t)B->P * sizeof(BState));
         if (e != hipSuccess) return fl_fail(FL_ERR_ALLOC, "batch solver workspace");
@@ -565,11 +566,16 @@ static int b_cg_body(fl_batch_s* B, float* u, float lam, bool matched, hipStream
     return FL_OK;
 }
 
-static int b_cg_run(fl_batch_s* B, const float* b, float* u, float lam, int iters, double tol, int flags,
+// The iterate lives in B->u while the solve runs, so one captured iteration
+// serves every call with the same lam and flags.
+static int b_cg_run(fl_batch_s* B, const float* b, float* u_caller, float lam, int iters, double tol, int flags,
                     hipStream_t st)
 {
     const bool matched = flags & FL_SOLVE_MATCHED;
     const unsigned g = (unsigned)std::min(B->vec_grid, B->vec_total);
+    const long long nv = B->voff[B->P];
+    float* u = B->u;
+    hipLaunchKernelGGL(k_bv_copy, dim3(1024), dim3(256), 0, st, u_caller, u, nv);
     int rc;
     if ((rc = b_normal(B, u, B->q, matched, st))) return rc;
     hipLaunchKernelGGL(k_bv_init, dim3(g), dim3(256), 0, st, B->q, u, b, lam, B->r, B->p, B->part, B->d_desc,
@@ -577,8 +583,9 @@ static int b_cg_run(fl_batch_s* B, const float* b, float* u, float lam, int iter
     hipLaunchKernelGGL(k_bfin_gamma, dim3(B->P), dim3(256), 0, st, B->part, B->d_vstart, B->st, tol, B->log,
                        B->maxlog);
     BCHK(hipGetLastError());
+    if (iters <= 0) return FL_OK;
     if (flags & FL_SOLVE_GRAPH) {
-        const bool hit = B->gexec && B->gkey == (const void*)u && B->gkey_lam == lam && B->gkey_flags == flags;
+        const bool hit = B->gexec && B->gkey_lam == lam && B->gkey_flags == flags;
         if (!hit) {
             b_drop_graph(B);
             BCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -592,7 +599,6 @@ static int b_cg_run(fl_batch_s* B, const float* b, float* u, float lam, int iter
             if (e != hipSuccess) return fl_hip_fail(e, "hipStreamEndCapture");
             B->graph = gr;
             BCHK(hipGraphInstantiate(&B->gexec, B->graph, nullptr, nullptr, 0));
-            B->gkey = u;
             B->gkey_lam = lam;
             B->gkey_flags = flags;
         }
@@ -601,6 +607,8 @@ static int b_cg_run(fl_batch_s* B, const float* b, float* u, float lam, int iter
         for (int k = 0; k < iters; k++)
             if ((rc = b_cg_body(B, u, lam, matched, st))) return rc;
     }
+    hipLaunchKernelGGL(k_bv_copy, dim3(1024), dim3(256), 0, st, u, u_caller, nv);
+    BCHK(hipGetLastError());
     return FL_OK;
 }
 

@@ -204,6 +204,7 @@ unsigned grid_for(size_t n)
 
 struct Workspace {
     float *r = nullptr, *p = nullptr, *q = nullptr, *sino = nullptr;
+    float* u = nullptr;      // the iterate while a solve runs (graphs capture this address)
     void* plo = nullptr;
     void* slo = nullptr;
     double* part = nullptr;
@@ -223,7 +224,6 @@ struct Workspace {
     // cached graph of one CG iteration
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
-    const void* gkey[4] = {nullptr, nullptr, nullptr, nullptr};
     float gkey_lam = 0.0f;
     int gkey_flags = -1;
 };
@@ -246,6 +246,15 @@ int lowp_type(int flags)
     return FL_F32;
 }
 
+void drop_graph(Workspace& w)
+{
+    if (w.gexec) (void)hipGraphExecDestroy(w.gexec);
+    if (w.graph) (void)hipGraphDestroy(w.graph);
+    w.gexec = nullptr;
+    w.graph = nullptr;
+    w.gkey_flags = -1;
+}
+
 int ensure_cg(fl_plan_s* pl, Workspace& w, int maxlog, int flags)
 {
     const size_t nx = (size_t)pl->nvox * pl->nt, ns = (size_t)pl->ndet * pl->nv;
@@ -253,6 +262,7 @@ int ensure_cg(fl_plan_s* pl, Workspace& w, int maxlog, int flags)
     if (!w.r) {
         if ((rc = ws_alloc((void**)&w.r, nx * 4, "r")) || (rc = ws_alloc((void**)&w.p, nx * 4, "p")) ||
             (rc = ws_alloc((void**)&w.q, nx * 4, "q")) || (rc = ws_alloc((void**)&w.sino, ns * 4, "sinogram")) ||
+            (rc = ws_alloc((void**)&w.u, nx * 4, "iterate")) ||
             (rc = ws_alloc((void**)&w.part, (NPART + 8) * sizeof(double), "partials")) ||
             (rc = ws_alloc((void**)&w.st, sizeof(CgState), "state")))
             return rc;
@@ -266,6 +276,7 @@ int ensure_cg(fl_plan_s* pl, Workspace& w, int maxlog, int flags)
             return rc;
     }
     if (maxlog > w.maxlog) {
+        drop_graph(w);   // the captured iteration holds the old log pointer and length
         if (w.log) (void)hipFree(w.log);
         w.log = nullptr;
         if ((rc = ws_alloc((void**)&w.log, (size_t)maxlog * sizeof(double), "log"))) return rc;
@@ -276,11 +287,10 @@ int ensure_cg(fl_plan_s* pl, Workspace& w, int maxlog, int flags)
 
 void free_ws(Workspace& w)
 {
-    void* ptrs[] = {w.r, w.p, w.q, w.sino, w.plo, w.slo, w.part, w.st, w.log, w.V, w.D, w.Atf, w.b, w.fpart};
+    void* ptrs[] = {w.r, w.p, w.q, w.sino, w.u, w.plo, w.slo, w.part, w.st, w.log, w.V, w.D, w.Atf, w.b, w.fpart};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
-    if (w.gexec) (void)hipGraphExecDestroy(w.gexec);
-    if (w.graph) (void)hipGraphDestroy(w.graph);
+    drop_graph(w);
     if (w.ev_in) (void)hipEventDestroy(w.ev_in);
     if (w.ev_out) (void)hipEventDestroy(w.ev_out);
     if (w.stream) (void)hipStreamDestroy(w.stream);
@@ -369,19 +379,22 @@ int cg_init(fl_plan_s* pl, Workspace& w, const float* b, float* u, float lam, in
 }
 
 // Runs `iters` CG iterations on w.stream (already ordered after the caller).
-int cg_run(fl_plan_s* pl, Workspace& w, const float* b, float* u, float lam, int iters, double tol, int flags)
+// The iterate lives in w.u while the solve runs, so one captured iteration
+// serves every call with the same lam and flags.
+int cg_run(fl_plan_s* pl, Workspace& w, const float* b, float* u_caller, float lam, int iters, double tol,
+           int flags)
 {
     hipStream_t st = w.stream;
+    const size_t n = (size_t)pl->nvox * pl->nt;
+    float* u = w.u;
+    HIPCHK(fl_launch_copy(u_caller, u, n, st));
     int rc;
     if ((rc = cg_init(pl, w, b, u, lam, flags, tol, st))) return rc;
     if (iters <= 0) return FL_OK;
     if (flags & FL_SOLVE_GRAPH) {
-        const bool hit = w.gexec && w.gkey[0] == (const void*)u && w.gkey_lam == lam && w.gkey_flags == flags;
+        const bool hit = w.gexec && w.gkey_lam == lam && w.gkey_flags == flags;
         if (!hit) {
-            if (w.gexec) (void)hipGraphExecDestroy(w.gexec);
-            if (w.graph) (void)hipGraphDestroy(w.graph);
-            w.gexec = nullptr;
-            w.graph = nullptr;
+            drop_graph(w);
             HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             rc = cg_body(pl, w, u, lam, flags, st);
             hipGraph_t gr = nullptr;
@@ -393,7 +406,6 @@ int cg_run(fl_plan_s* pl, Workspace& w, const float* b, float* u, float lam, int
             if (e != hipSuccess) return fl_hip_fail(e, "hipStreamEndCapture");
             w.graph = gr;
             HIPCHK(hipGraphInstantiate(&w.gexec, w.graph, nullptr, nullptr, 0));
-            w.gkey[0] = u;
             w.gkey_lam = lam;
             w.gkey_flags = flags;
         }
@@ -402,6 +414,7 @@ int cg_run(fl_plan_s* pl, Workspace& w, const float* b, float* u, float lam, int
         for (int k = 0; k < iters; k++)
             if ((rc = cg_body(pl, w, u, lam, flags, st))) return rc;
     }
+    HIPCHK(fl_launch_copy(u, u_caller, n, st));
     return FL_OK;
 }
 

@@ -135,6 +135,28 @@ def test_cgls_low_precision_storage(gpu, oracle):
         assert rel < tol
 
 
+def test_cgls_graph_reuse_across_calls(gpu):
+    """A graph captured by a short solve is reused by a longer one (the log
+    buffer grows in between) and by a solve on another iterate tensor: every
+    graph-replayed result and gamma log equals the eager one bit for bit."""
+    _say("case: CGLS graph reuse")
+    from framelab_amd.phantoms import ellipses_2d
+    from framelab_amd.projector import Projector
+    from framelab_amd.solvers import cgls
+    p = small_fan(nd=96, nv=40, N=32)
+    P = Projector(p)
+    x = torch.from_numpy(ellipses_2d(32, 5, seed=8).astype(np.float32)).to(gpu)
+    b = P.adjoint(P.forward(x))
+    u0 = torch.zeros(P.vol_shape, device=gpu)
+    cgls(P, b, u0, 0.01, 1, graph=True)
+    for iters in (6, 9):
+        ug, lg, dg = cgls(P, b, u0 + 0.0, 0.01, iters, graph=True)
+        ue, le, de = cgls(P, b, u0, 0.01, iters, graph=False)
+        assert dg == de == iters
+        assert np.array_equal(lg, le), (lg, le)
+        assert torch.equal(ug, ue)
+
+
 def test_split_bregman_vs_oracle(gpu, oracle):
     """Fan 48^2, linear framelet, mu=1, lam=1e-2, 4 outer x 5 inner: relative
     L2 difference to the FP64 oracle below 2e-3."""
