@@ -91,6 +91,8 @@ SIGNATURES = {
     "fl_grad_data": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(C.c_double), C.c_int, _vp]),
     "fl_grad_framelet": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_float, C.c_float,
                                    C.c_int, _vp, _vp, C.POINTER(C.c_double), _vp]),
+    "fl_plan_autotune": (C.c_int, [_vp, _ip, _fp]),
+    "fl_plan_set_tile": (C.c_int, [_vp, C.c_int]),
     "fl_batch_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.POINTER(FlGeometry)]),
     "fl_batch_destroy": (C.c_int, [_vp]),
     "fl_batch_layout": (C.c_int, [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),

@@ -71,7 +71,7 @@ static int upload(T** dst, const T* src, size_t n, const char* what)
 static void plan_free_arrays(fl_plan_s* p)
 {
     void* ptrs[] = {p->d_cs, p->d_sn, p->d_sdz, p->d_ydet, p->d_zdet, p->d_cphi, p->d_sphi,
-                    p->d_cdet, p->d_sdet, p->d_slot_view, p->d_view_frame, p->d_idY, p->d_Nv};
+                    p->d_cdet, p->d_sdet, p->d_slot_view, p->d_view_frame, p->d_idY, p->d_Nv, p->d_bricks};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
 }
@@ -193,13 +193,26 @@ extern "C" int fl_plan_create(fl_plan* out, const fl_geometry* g)
         delete p;
         return rc;
     }
+    if (cone) {
+        // the forward gathers from a bricked copy of its input (8x8x1 bricks,
+        // padded to whole bricks; 4-byte elements cover every storage type)
+        const size_t fr = (size_t)((g->nx + 7) / 8) * ((g->ny + 7) / 8) * 64 * (size_t)g->nz;
+        if (hipMalloc(&p->d_bricks, fr * (size_t)g->nt * 4) != hipSuccess) rc = fl_fail(FL_ERR_ALLOC, "brick buffer");
+    }
+    if (rc) {
+        plan_free_arrays(p);
+        delete p;
+        return rc;
+    }
 
     FwdGeom& f = p->fg;
     f.cs = p->d_cs; f.sn = p->d_sn; f.sdz = p->d_sdz; f.ydet = p->d_ydet; f.zdet = p->d_zdet;
     f.slot_view = p->d_slot_view; f.view_frame = p->d_view_frame;
     f.SO = g->SO; f.OD = g->OD; f.scale = g->scale; f.dz0 = cone ? g->dz : 1.0f;
     f.nx = g->nx; f.ny = g->ny; f.nz = cone ? g->nz : 0; f.na = g->na; f.nb = g->nb;
-    f.nslots = g->nv; f.nvox = (size_t)nvox;
+    f.nslots = g->nv; f.nvox = (size_t)nvox; f.nframes = g->nt;
+    f.bricks = p->d_bricks;
+    f.tile = cone ? 1 : 0;
 
     AdjGeom& a = p->ag;
     a.cphi = p->d_cphi; a.sphi = p->d_sphi; a.cdet = p->d_cdet; a.sdet = p->d_sdet;
@@ -260,6 +273,68 @@ extern "C" int fl_plan_sizes(fl_plan p, long long* nvox, int* nt, long long* nde
 }
 
 static bool dtype_ok(int t) { return t == FL_F32 || t == FL_BF16 || t == FL_F16; }
+
+// Times every cone-forward ray-tile shape on this plan's geometry (zero
+// volume, FP32; the traversal does not depend on the values) and keeps the
+// fastest.  Fan plans have one shape.
+extern "C" int fl_plan_autotune(fl_plan p, int* fwd_tile, float* fwd_ms)
+{
+    if (!p) return fl_fail(FL_ERR_INVALID, "null plan");
+    if (!p->cone) {
+        if (fwd_tile) *fwd_tile = 0;
+        if (fwd_ms) *fwd_ms = 0.0f;
+        return fl_ok();
+    }
+    const size_t nxb = (size_t)p->nvox * p->nt * sizeof(float), nyb = (size_t)p->ndet * p->nv * sizeof(float);
+    float *x = nullptr, *y = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = FL_OK;
+    if (hipMalloc((void**)&x, nxb) != hipSuccess || hipMalloc((void**)&y, nyb) != hipSuccess)
+        rc = fl_fail(FL_ERR_ALLOC, "autotune buffers (%zu + %zu bytes)", nxb, nyb);
+    if (!rc && (hipStreamCreate(&st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+                hipEventCreate(&e1) != hipSuccess))
+        rc = fl_fail(FL_ERR_HIP, "autotune stream/events");
+    int best = p->fg.tile;
+    float best_ms = 1e30f;
+    if (!rc) {
+        hipError_t e = hipMemsetAsync(x, 0, nxb, st);
+        FwdGeom g = p->fg;
+        for (int t = 0; t < FL_FWD_NTILES && e == hipSuccess; t++) {
+            g.tile = t;
+            e = fl_launch_fwd_cone(x, FL_F32, y, FL_F32, g, st);
+            if (e == hipSuccess) e = hipEventRecord(e0, st);
+            if (e == hipSuccess) e = fl_launch_fwd_cone(x, FL_F32, y, FL_F32, g, st);
+            if (e == hipSuccess) e = hipEventRecord(e1, st);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e == hipSuccess && ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
+        }
+        if (e != hipSuccess) rc = fl_hip_fail(e, "autotune forward");
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (x) (void)hipFree(x);
+    if (y) (void)hipFree(y);
+    if (rc) return rc;
+    p->fg.tile = best;
+    if (fwd_tile) *fwd_tile = best;
+    if (fwd_ms) *fwd_ms = best_ms;
+    return fl_ok();
+}
+
+// Selects a cone-forward ray-tile shape by index (0..5).
+extern "C" int fl_plan_set_tile(fl_plan p, int tile)
+{
+    if (!p || tile < 0 || tile >= FL_FWD_NTILES) return fl_fail(FL_ERR_INVALID, "bad plan or tile %d", tile);
+    p->fg.tile = tile;
+    return fl_ok();
+}
 
 extern "C" int fl_forward(fl_plan p, const void* d_x, int xt, void* d_y, int yt, void* stream)
 {

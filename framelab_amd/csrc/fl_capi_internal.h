@@ -19,10 +19,13 @@ struct fl_plan_s {
     float *d_cs = nullptr, *d_sn = nullptr, *d_sdz = nullptr, *d_ydet = nullptr, *d_zdet = nullptr;
     float *d_cphi = nullptr, *d_sphi = nullptr, *d_cdet = nullptr, *d_sdet = nullptr;
     int *d_slot_view = nullptr, *d_view_frame = nullptr, *d_idY = nullptr, *d_Nv = nullptr;
+    void* d_bricks = nullptr;   // cone: bricked copy of the forward input (nt frames, 8x8 bricks, 4-byte)
     // lazily allocated workspace (solvers, gradients)
     void* ws = nullptr;
     size_t ws_bytes = 0;
 };
+
+constexpr int FL_FWD_NTILES = 6;   // cone forward ray-tile shapes (siddon.hip)
 
 int fl_fail(int code, const char* fmt, ...);
 int fl_ok();

@@ -13,10 +13,12 @@ struct FwdGeom {
     const int* slot_view;   // output slot -> view index
     const int* view_frame;  // view -> image frame (id_X)
     float SO, OD, scale, dz0;
-    int nx, ny, nz, na, nb, nslots;
+    int nx, ny, nz, na, nb, nslots, nframes;
     size_t nvox;            // voxels per frame
     // detector index maps for voxel shadows: cell a is at y_det[0] + a*pitch
     float ya0, inv_pa, zb0, inv_pb;
+    int tile;               // cone forward launch shape, 0..FL_FWD_NTILES-1
+    void* bricks;           // cone: bricked copy of the input frames (plan-owned)
 };
 
 // Voxel-driven adjoint geometry (the reference para tables).

@@ -79,6 +79,17 @@ FL_HD int sid_count(const SidAxis& a, float t, bool strict)
     return j;
 }
 
+// num / d, correctly rounded, from r = 1/d correctly rounded: q = num*r is
+// within an ulp of the quotient, the fused residual num - q*d is exact, and
+// q + residual*r rounds to the IEEE quotient (Markstein's theorem; holds
+// without overflow or underflow, which plane parameters never reach).
+FL_HD float sid_div(float num, float d, float r)
+{
+    const float q = num * r;
+    const float e = fmaf(-q, d, num);
+    return fmaf(e, r, q);
+}
+
 // Cell index of the midpoint along an x/y axis: floor(o + m*d + n/2).
 FL_HD int sid_cell_xy(const SidAxis& a, float m)
 {
@@ -104,14 +115,49 @@ FL_HD bool sid_far(const SidAxis& a, int t)
     return a.on && (a.st > 0 ? t >= a.n : t < 0);
 }
 
-// Generic walk over one ray.  NZ3 = true for the cone (3 axes).  The
-// visitor receives (linear voxel index (z*ny+y)*nx+x, alpha difference of
-// the segment) for every in-volume segment in traversal order; the length
-// is that difference times |d|.
-template <bool NZ3, class Visit>
-FL_HD void sid_walk(const SidAxis& ax, const SidAxis& ay, const SidAxis& az,
-                                         float L, bool dz_one, Visit&& visit)
+// Walk state of one axis: parameter of the next plane (INF once past the
+// last), that plane's coordinate index as a float (exact integer), and the
+// constants of the branch-free advance.
+struct SidNext {
+    float a;     // alpha of the next plane
+    float p;     // its index i (float)
+    float st;    // +1 / -1
+    float past;  // index one beyond the last plane
+    float r;     // 1/d, correctly rounded
+};
+
+FL_HD void sid_next_init(SidNext& n, const SidAxis& ax, int j)
 {
+    n.st = (float)ax.st;
+    n.p = (float)(ax.i0 + ax.st * j);
+    n.past = (float)(ax.i0 + ax.st * (ax.n + 1));
+    n.r = ax.on ? 1.0f / ax.d : 0.0f;
+    n.a = (ax.on && j <= ax.n) ? sid_alpha(ax, j) : FL_INF;
+}
+
+// If this axis's next plane is the one just reached (alpha == an), move to
+// the following plane.  Branch-free; exact ties advance every tied axis.
+template <bool S1>
+FL_HD void sid_next_step(SidNext& n, const SidAxis& ax, float an)
+{
+    const float pn = n.p + n.st;
+    const float num = (S1 ? pn : pn * ax.s) - ax.o;
+    float v = sid_div(num, ax.d, n.r);
+    v = (pn == n.past) ? FL_INF : v;
+    const bool adv = n.a == an;
+    n.a = adv ? v : n.a;
+    n.p = adv ? pn : n.p;
+}
+
+// Generic walk over one ray.  NZ3 = true for the cone (3 axes); DZ1 = the
+// z plane spacing is 1 (cell index needs no division).  The visitor
+// receives (linear voxel index (z*ny+y)*nx+x, alpha difference of the
+// segment) for every in-volume segment in traversal order; the length is
+// that difference times |d|.
+template <bool NZ3, bool DZ1, bool XYZ, class Visit>
+FL_HD void sid_walk_t(const SidAxis& ax, const SidAxis& ay, const SidAxis& az, float L, Visit&& visit)
+{
+    constexpr bool dz_one = DZ1;
     // Inactive axes keep a constant cell index; if it is outside, nothing
     // of the ray is inside the volume.
     if (!ax.on && !sid_in(ax, sid_cell_xy(ax, 0.5f))) return;
@@ -142,31 +188,55 @@ FL_HD void sid_walk(const SidAxis& ax, const SidAxis& ay, const SidAxis& az,
     }
 
     // Forward walk from `cur`.
-    int jx = 0, jy = 0, jz = 0;
-    float nx_ = FL_INF, ny_ = FL_INF, nz_ = FL_INF;
-    if (ax.on) { jx = sid_count(ax, cur, false); if (jx <= ax.n) nx_ = sid_alpha(ax, jx); }
-    if (ay.on) { jy = sid_count(ay, cur, false); if (jy <= ay.n) ny_ = sid_alpha(ay, jy); }
-    if (NZ3 && az.on) { jz = sid_count(az, cur, false); if (jz <= az.n) nz_ = sid_alpha(az, jz); }
+    SidNext qx, qy, qz;
+    sid_next_init(qx, ax, ax.on ? sid_count(ax, cur, false) : 0);
+    sid_next_init(qy, ay, ay.on ? sid_count(ay, cur, false) : 0);
+    if (NZ3) sid_next_init(qz, az, az.on ? sid_count(az, cur, false) : 0);
+    else qz.a = FL_INF;
+    const float rs = NZ3 && !DZ1 ? 1.0f / az.s : 1.0f;
 
     const int nx = ax.n, ny = ay.n;
     for (;;) {
-        float an = fminf(nx_, ny_);
-        if (NZ3) an = fminf(an, nz_);
+        float an = fminf(qx.a, qy.a);
+        if (NZ3) an = fminf(an, qz.a);
         if (an == FL_INF) break;
         float m = (cur + an) * 0.5f;
         int tx = sid_cell_xy(ax, m);
         int ty = sid_cell_xy(ay, m);
-        int tz = NZ3 ? sid_cell_z(az, m, dz_one) : 0;
+        int tz = 0;
+        if (NZ3) {
+            float t = az.o + m * az.d;
+            if (!DZ1) t = sid_div(t, az.s, rs);
+            tz = (int)floorf(t + (float)az.nh);
+        }
         bool in = sid_in(ax, tx) && sid_in(ay, ty) && (!NZ3 || sid_in(az, tz));
         if (in) {
-            visit((tz * ny + ty) * nx + tx, an - cur);
+            if constexpr (XYZ) visit(tx, ty, tz, an - cur);
+            else visit((tz * ny + ty) * nx + tx, an - cur);
         } else {
             bool far = sid_far(ax, tx) || sid_far(ay, ty) || (NZ3 && sid_far(az, tz));
             if (far) break;
         }
-        if (nx_ == an) { jx++; nx_ = jx <= ax.n ? sid_alpha(ax, jx) : FL_INF; }
-        if (ny_ == an) { jy++; ny_ = jy <= ay.n ? sid_alpha(ay, jy) : FL_INF; }
-        if (NZ3 && nz_ == an) { jz++; nz_ = jz <= az.n ? sid_alpha(az, jz) : FL_INF; }
+        sid_next_step<true>(qx, ax, an);
+        sid_next_step<true>(qy, ay, an);
+        if (NZ3) sid_next_step<DZ1>(qz, az, an);
         cur = an;
     }
+}
+
+template <bool NZ3, class Visit>
+FL_HD void sid_walk(const SidAxis& ax, const SidAxis& ay, const SidAxis& az, float L, bool dz_one, Visit&& visit)
+{
+    if (dz_one) sid_walk_t<NZ3, true, false>(ax, ay, az, L, visit);
+    else sid_walk_t<NZ3, false, false>(ax, ay, az, L, visit);
+}
+
+// As sid_walk, but the visitor receives the cell coordinates (tx, ty, tz)
+// instead of the linear index.
+template <bool NZ3, class Visit>
+FL_HD void sid_walk_xyz(const SidAxis& ax, const SidAxis& ay, const SidAxis& az, float L, bool dz_one,
+                        Visit&& visit)
+{
+    if (dz_one) sid_walk_t<NZ3, true, true>(ax, ay, az, L, visit);
+    else sid_walk_t<NZ3, false, true>(ax, ay, az, L, visit);
 }

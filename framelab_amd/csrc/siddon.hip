@@ -9,6 +9,8 @@
 // Ax_fan_mf_cpu_siddon.cpp:92-151 operation by operation (this file is built
 // with -ffp-contract=off), so ray sums are bit-identical to the reference CPU
 // path; see fl_siddon.h for the traversal.
+#include <algorithm>
+
 #include "fl_common.h"
 #include "fl_geom.h"
 #include "fl_siddon.h"
@@ -60,21 +62,59 @@ __device__ __forceinline__ RayGeo fan_ray(const FwdGeom& g, int iv, int id)
 
 }  // namespace
 
-template <typename VT, typename YT, int TA, int TB>
+// Bricked volume layout for the ray gathers: the frame is cut into
+// BX x BY x 1 bricks stored contiguously (x fastest inside a brick, bricks
+// x-major), so a cache line holds a compact patch of voxels whatever the
+// view angle and neighbouring rays share lines.  BX = BY = 1 is the plain
+// layout.  Padding to whole bricks is never read.
+template <int BX, int BY>
+struct Bricks {
+    int nbx, nby;
+    size_t frame;   // elements per bricked frame
+    __host__ __device__ Bricks(int nx, int ny, int nz)
+        : nbx((nx + BX - 1) / BX), nby((ny + BY - 1) / BY), frame((size_t)nbx * nby * BX * BY * nz) {}
+    __device__ __forceinline__ unsigned at(int tx, int ty, int tz) const
+    {
+        if (BX == 1 && BY == 1) return ((unsigned)tz * nby + ty) * nbx + tx;
+        return (((unsigned)tz * nby + ty / BY) * nbx + tx / BX) * (BX * BY) + (ty % BY) * BX + tx % BX;
+    }
+};
+
+template <typename VT, int BX, int BY>
+__global__ __launch_bounds__(256) void k_to_bricks(const VT* __restrict__ x, VT* __restrict__ xb, int nx, int ny,
+                                                  int nz, size_t nvox, size_t n)
+{
+    const Bricks<BX, BY> br(nx, ny, nz);
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const size_t f = i / nvox, v = i % nvox;
+        const int tx = (int)(v % nx), ty = (int)((v / nx) % ny), tz = (int)(v / ((size_t)nx * ny));
+        xb[f * br.frame + br.at(tx, ty, tz)] = x[i];
+    }
+}
+
+// A workgroup covers a TA x TB patch of detector cells of one view; each
+// wave covers a WA x (64/WA) sub-patch, so that its 64 rays run close
+// together through the volume and their gathers share cache lines.  X is
+// the input in the BX x BY brick layout.
+template <typename VT, typename YT, int TA, int TB, int WA, int BX, int BY>
 __global__ __launch_bounds__(TA* TB) void k_fwd_cone(const VT* __restrict__ X, YT* __restrict__ Y,
                                                     FwdGeom g)
 {
+    constexpr int WB = 64 / WA, NWA = TA / WA;
+    static_assert(TA % WA == 0 && TB % WB == 0 && (TA * TB) % 64 == 0, "tile shape");
     const uint32_t tiles_a = (g.na + TA - 1) / TA;
     const uint32_t tiles_b = (g.nb + TB - 1) / TB;
     const uint32_t tpv = tiles_a * tiles_b;
     const uint32_t b = fl_xcd_remap(blockIdx.x, gridDim.x);
     const int slot = (int)(b / tpv);
     const uint32_t t = b % tpv;
-    const int ia = (int)(t % tiles_a) * TA + (int)threadIdx.x;
-    const int ib = (int)(t / tiles_a) * TB + (int)threadIdx.y;
+    const int w = (int)threadIdx.x >> 6, l = (int)threadIdx.x & 63;
+    const int ia = (int)(t % tiles_a) * TA + (w % NWA) * WA + (l % WA);
+    const int ib = (int)(t / tiles_a) * TB + (w / NWA) * WB + (l / WA);
     if (slot >= g.nslots || ia >= g.na || ib >= g.nb) return;
     const int iv = g.slot_view[slot];
-    const VT* __restrict__ x = X + (size_t)g.view_frame[iv] * g.nvox;
+    const Bricks<BX, BY> br(g.nx, g.ny, g.nz);
+    const VT* __restrict__ x = X + (size_t)g.view_frame[iv] * br.frame;
 
     RayGeo r = cone_ray(g, iv, ia, ib);
     SidAxis ax, ay, az;
@@ -83,9 +123,9 @@ __global__ __launch_bounds__(TA* TB) void k_fwd_cone(const VT* __restrict__ X, Y
     sid_axis_init(az, r.z1, r.dz, g.dz0, g.nz);
     const float L = r.L;
     float acc = 0.0f;
-    sid_walk<true>(ax, ay, az, L, g.dz0 == 1.0f, [&](int J, float dal) {
+    sid_walk_xyz<true>(ax, ay, az, L, g.dz0 == 1.0f, [&](int tx, int ty, int tz, float dal) {
         float Si = dal * L;
-        acc = acc + Si * fl_load(x, (size_t)J);
+        acc = acc + Si * fl_load(x, (size_t)br.at(tx, ty, tz));
     });
     fl_store(Y, ((size_t)slot * g.nb + ib) * g.na + ia, acc * g.scale);
 }
@@ -134,16 +174,41 @@ __global__ void k_trace(FwdGeom g, int cone, int nrays, const int* __restrict__ 
 
 // ---- host launchers ------------------------------------------------------
 
-template <typename VT, typename YT>
-static hipError_t launch_cone_t(const void* X, void* Y, const FwdGeom& g, hipStream_t st)
+template <typename VT, typename YT, int TA, int TB, int WA, int BX, int BY>
+static hipError_t launch_cone_shape(const void* X, void* Y, const FwdGeom& g, hipStream_t st)
 {
-    constexpr int TA = 64, TB = 4;
     uint32_t tiles = ((g.na + TA - 1) / TA) * ((g.nb + TB - 1) / TB);
     uint32_t grid = tiles * (uint32_t)g.nslots;
     if (grid == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_fwd_cone<VT, YT, TA, TB>), dim3(grid), dim3(TA, TB), 0, st,
-                       (const VT*)X, (YT*)Y, g);
+    const VT* src = (const VT*)X;
+    if (BX > 1 || BY > 1) {
+        // nt frames into the plan's brick buffer (sized for 4-byte elements)
+        const int nt = g.nframes;
+        const size_t n = g.nvox * (size_t)nt;
+        const unsigned gb = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+        hipLaunchKernelGGL((k_to_bricks<VT, BX, BY>), dim3(gb), dim3(256), 0, st, (const VT*)X, (VT*)g.bricks,
+                           g.nx, g.ny, g.nz, g.nvox, n);
+        src = (const VT*)g.bricks;
+    }
+    hipLaunchKernelGGL((k_fwd_cone<VT, YT, TA, TB, WA, BX, BY>), dim3(grid), dim3(TA * TB), 0, st, src, (YT*)Y,
+                       g);
     return hipGetLastError();
+}
+
+// Launch shapes, indexed by FwdGeom::tile (fl_plan_autotune times them):
+// workgroup TA x TB rays, waves WA x 64/WA, volume bricks BX x BY x 1.
+template <typename VT, typename YT>
+static hipError_t launch_cone_t(const void* X, void* Y, const FwdGeom& g, hipStream_t st)
+{
+    if (g.tile != 0 && g.tile != 3 && !g.bricks) return hipErrorInvalidValue;
+    switch (g.tile) {
+    case 0: return launch_cone_shape<VT, YT, 64, 4, 64, 1, 1>(X, Y, g, st);
+    case 1: return launch_cone_shape<VT, YT, 16, 16, 8, 8, 8>(X, Y, g, st);
+    case 2: return launch_cone_shape<VT, YT, 32, 8, 8, 8, 8>(X, Y, g, st);
+    case 3: return launch_cone_shape<VT, YT, 16, 16, 8, 1, 1>(X, Y, g, st);
+    case 4: return launch_cone_shape<VT, YT, 16, 16, 8, 4, 4>(X, Y, g, st);
+    default: return launch_cone_shape<VT, YT, 64, 4, 64, 8, 8>(X, Y, g, st);
+    }
 }
 
 template <typename VT, typename YT>

@@ -143,6 +143,17 @@ class Projector:
                                          _stream()))
         return out
 
+    def autotune(self):
+        """Time the cone-forward ray-tile shapes on this geometry and keep the fastest.
+        Returns (tile index, milliseconds of one forward with it)."""
+        t, ms = C.c_int(0), C.c_float(0.0)
+        _lib.check(_lib.lib().fl_plan_autotune(self._h, C.byref(t), C.byref(ms)))
+        return t.value, ms.value
+
+    def set_tile(self, tile: int):
+        """Select a cone-forward ray-tile shape by index (0..5)."""
+        _lib.check(_lib.lib().fl_plan_set_tile(self._h, int(tile)))
+
     def trace(self, rays, cap: int | None = None):
         """Traversal log (voxel index, length) of (view, ia, ib) rays, for parity tests."""
         rays = np.ascontiguousarray(np.asarray(rays, dtype=np.int32).reshape(-1, 3))

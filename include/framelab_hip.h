@@ -113,6 +113,12 @@ int fl_plan_destroy(fl_plan plan);
 /* sizes: voxels per frame, frames, detector cells per view, views/slots */
 int fl_plan_sizes(fl_plan plan, long long *nvox, int *nt, long long *ndet, int *nv);
 
+/* Times the cone-forward ray-tile shapes on this plan's geometry and keeps
+ * the fastest (outputs optional; fan plans: no-op).  fl_plan_set_tile picks
+ * a shape by index 0..5.  No reference counterpart (launch tuning). */
+int fl_plan_autotune(fl_plan plan, int *fwd_tile, float *fwd_ms);
+int fl_plan_set_tile(fl_plan plan, int tile);
+
 /* adjoint kinds */
 #define FL_ADJ_EXACT 0   /* reference exact-chord voxel-driven adjoint (find_l / find_l_3d) */
 #define FL_ADJ_MATCHED 1 /* voxel-driven transpose of the Siddon forward (same lengths) */
