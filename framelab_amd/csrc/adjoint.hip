@@ -24,62 +24,6 @@ __global__ __launch_bounds__(256) void k_adj_fan(const YT* __restrict__ Y, XT* _
     fl_store(X, (size_t)it * g.nvox + (size_t)iy * g.nx + ix, fl_fan_adj_pixel(Y, g, it, ix, iy));
 }
 
-// Cone: a workgroup is an 8x8x4 voxel brick, so the 256 threads gather from
-// one compact detector footprint per view (L1-resident).
-template <typename YT, typename XT>
-__global__ __launch_bounds__(256) void k_adj_cone(const YT* __restrict__ Y, XT* __restrict__ X, AdjGeom g)
-{
-    const int bx = (g.nx + 7) / 8, by = (g.ny + 7) / 8, bz = (g.nz + 3) / 4;
-    const uint32_t nblk = (uint32_t)bx * by * bz;
-    const uint32_t b = fl_xcd_remap(blockIdx.x, nblk);
-    const int it = blockIdx.y;
-    const int tbx = (int)(b % bx), tby = (int)((b / bx) % by), tbz = (int)(b / ((uint32_t)bx * by));
-    const int ix = tbx * 8 + (threadIdx.x & 7);
-    const int iy = tby * 8 + ((threadIdx.x >> 3) & 7);
-    const int iz = tbz * 4 + (threadIdx.x >> 6);
-    if (ix >= g.nx || iy >= g.ny || iz >= g.nz) return;
-
-    const int na = g.na, nb = g.nb, na2 = na / 2, nb2 = nb / 2;
-    const float SD = g.SO + g.OD;
-    const float d = sqrtf((1 + g.dz * g.dz) / 2);
-    const float zc = ((float)iz + 0.5f - (float)(g.nz / 2)) * g.dz;
-    const float yc = (float)iy + 0.5f - (float)(g.ny / 2);
-    const float xc = (float)ix + 0.5f - (float)(g.nx / 2);
-    float acc = 0.0f;
-    const int nvf = g.Nv[it];
-    for (int j = 0; j < nvf; j++) {
-        const int iv = g.id_Y[it * g.tmp_size + j];
-        const YT* __restrict__ y = Y + (size_t)iv * g.nd;
-        const float c = g.cphi[iv], s = g.sphi[iv];
-        const float sz = g.sdz[iv];
-        const float xr = c * xc + s * yc;
-        const float yr = -s * xc + c * yc;
-        const float zr = zc - sz;
-        float t = SD / ((xr + g.SO) * g.dy_det);
-        int ahi = (int)ceilf((yr + 1) * t - g.y_os + (float)na2);
-        int alo = (int)floorf((yr - 1) * t - g.y_os + (float)na2);
-        t = SD / ((xr + g.SO) * g.dz_det);
-        int bhi = (int)ceilf((zr + d) * t + (float)nb2);
-        int blo = (int)floorf((zr - d) * t + (float)nb2);
-        if (alo < 0) alo = 0;
-        if (ahi > na - 1) ahi = na - 1;
-        if (blo < 0) blo = 0;
-        if (bhi > nb - 1) bhi = nb - 1;
-        const float x1 = c * (-g.SO), y1 = s * (-g.SO), z1 = sz;
-        for (int ib = blo; ib <= bhi; ib++) {
-            const float z2 = g.zdet[ib] + sz;
-            for (int ia = alo; ia <= ahi; ia++) {
-                const float yd = g.ydet[ia];
-                const float x2 = c * g.OD - s * yd;
-                const float y2 = s * g.OD + c * yd;
-                const float l = fl_chord_box3(x1, y1, z1, x2, y2, z2, 1.0f, 1.0f, g.dz, xc, yc, zc);
-                acc = acc + l * fl_load(y, (size_t)ib * na + ia);
-            }
-        }
-    }
-    fl_store(X, (size_t)it * g.nvox + ((size_t)iz * g.ny + iy) * g.nx + ix, acc * g.scale);
-}
-
 // Cone, footprint-staged: per view the workgroup stages its detector
 // footprint in LDS -- the rays' direction components, the six direction
 // ratios find_l_3d divides out (each computed once per workgroup by the same
@@ -88,10 +32,12 @@ __global__ __launch_bounds__(256) void k_adj_cone(const YT* __restrict__ Y, XT* 
 // more than 0.02 voxel outside the box, where the float face tests of the
 // chord routine cannot succeed) and evaluates the chord of the rest with the
 // reference arithmetic, summing in the reference order.  Skipped pairs are
-// exact zeros, so the result is bit-identical to k_adj_cone.
+// exact zeros (the reference adds l*y = +-0 for them, which leaves a sum
+// that starts at +0 unchanged), so the result is bit-identical to the
+// reference kernel.
 namespace {
 constexpr int FP_A = 48;    // footprint columns held in LDS
-constexpr int FP_B = 24;    // footprint rows
+constexpr int FP_B = 32;    // footprint rows
 constexpr float MISS_MARGIN = 0.02f;
 
 __device__ __forceinline__ int wave_min(int v)
@@ -106,8 +52,15 @@ __device__ __forceinline__ int wave_max(int v)
 }
 }  // namespace
 
-template <typename YT, typename XT>
-__global__ __launch_bounds__(256) void k_adj_cone_fp(const YT* __restrict__ Y, XT* __restrict__ X, AdjGeom g)
+// Exact cone adjoint, z-column form.  A workgroup is an 8 x 8 x (4*KZ)
+// voxel brick; each thread owns KZ z-adjacent voxels of one (x, y) column,
+// which share the view rotation, the detector-column window and the
+// workgroup's per-view footprint tables in LDS.  Per voxel, the cells and
+// the order of accumulation are those of the reference kernel
+// (Atx_cone_mf_gpu_new.cu:58-86: views in order, rows outer, columns
+// inner), restricted to the cells whose chord can be non-zero.
+template <typename YT, typename XT, int KZ>
+__global__ __launch_bounds__(256) void k_adj_cone_zc(const YT* __restrict__ Y, XT* __restrict__ X, AdjGeom g)
 {
     __shared__ float c_a[FP_A], c_b[FP_A], c_sba[FP_A], c_sab[FP_A];
     __shared__ float c_nx[FP_A], c_ny[FP_A], c_off[FP_A], c_hs[FP_A];
@@ -117,57 +70,77 @@ __global__ __launch_bounds__(256) void k_adj_cone_fp(const YT* __restrict__ Y, X
     __shared__ float q_y[FP_A * FP_B];
     __shared__ int red[2][4][4];
 
-    const int bx = (g.nx + 7) / 8, by = (g.ny + 7) / 8, bz = (g.nz + 3) / 4;
+    constexpr int BZ = 4 * KZ;
+    const int bx = (g.nx + 7) / 8, by = (g.ny + 7) / 8, bz = (g.nz + BZ - 1) / BZ;
     const uint32_t nblk = (uint32_t)bx * by * bz;
     const uint32_t b = fl_xcd_remap(blockIdx.x, nblk);
     const int it = blockIdx.y;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int ix = (int)(b % bx) * 8 + (tid & 7);
     const int iy = (int)((b / bx) % by) * 8 + ((tid >> 3) & 7);
-    const int iz = (int)(b / ((uint32_t)bx * by)) * 4 + (tid >> 6);
-    const bool active = ix < g.nx && iy < g.ny && iz < g.nz;
+    const int iz0 = (int)(b / ((uint32_t)bx * by)) * BZ + wv * KZ;
+    const bool col = ix < g.nx && iy < g.ny;
 
     const int na = g.na, nb = g.nb, na2 = na / 2, nb2 = nb / 2;
     const float SD = g.SO + g.OD;
     const float d = sqrtf((1 + g.dz * g.dz) / 2);
-    const float zc = ((float)iz + 0.5f - (float)(g.nz / 2)) * g.dz;
     const float yc = (float)iy + 0.5f - (float)(g.ny / 2);
     const float xc = (float)ix + 0.5f - (float)(g.nx / 2);
     const float hz = g.dz / 2;
-    float acc = 0.0f;
+    float zc[KZ], acc[KZ];
+#pragma unroll
+    for (int k = 0; k < KZ; k++) {
+        zc[k] = ((float)(iz0 + k) + 0.5f - (float)(g.nz / 2)) * g.dz;
+        acc[k] = 0.0f;
+    }
     const int nvf = g.Nv[it];
     for (int j = 0; j < nvf; j++) {
         const int iv = g.id_Y[it * g.tmp_size + j];
         const YT* __restrict__ y = Y + (size_t)iv * g.nd;
         const float c = g.cphi[iv], s = g.sphi[iv];
         const float sz = g.sdz[iv];
-        int alo = na, ahi = -1, blo = nb, bhi = -1;
-        if (active) {
+        int alo = na, ahi = -1;
+        int blo[KZ], bhi[KZ];
+        int ublo = nb, ubhi = -1;
+#pragma unroll
+        for (int k = 0; k < KZ; k++) { blo[k] = nb; bhi[k] = -1; }
+        if (col) {
             const float xr = c * xc + s * yc;
             const float yr = -s * xc + c * yc;
-            const float zr = zc - sz;
             float t = SD / ((xr + g.SO) * g.dy_det);
             ahi = (int)ceilf((yr + 1) * t - g.y_os + (float)na2);
             alo = (int)floorf((yr - 1) * t - g.y_os + (float)na2);
-            t = SD / ((xr + g.SO) * g.dz_det);
-            bhi = (int)ceilf((zr + d) * t + (float)nb2);
-            blo = (int)floorf((zr - d) * t + (float)nb2);
             if (alo < 0) alo = 0;
             if (ahi > na - 1) ahi = na - 1;
-            if (blo < 0) blo = 0;
-            if (bhi > nb - 1) bhi = nb - 1;
-            // Narrow to the cells whose ray centre meets the voxel's shadow
-            // (plus a twentieth of a cell): every cell outside it has a zero
-            // chord, so the sum over the intersection is the reference sum.
-            const ShadowWin sw = fl_shadow_window(c, s, g.SO, SD, xc, yc, zr, hz, g.ya0, g.inv_pa, g.zb0,
-                                                  g.inv_pb, 0.05f, na, nb);
-            alo = max(alo, sw.alo); ahi = min(ahi, sw.ahi);
-            blo = max(blo, sw.blo); bhi = min(bhi, sw.bhi);
-            if (alo > ahi || blo > bhi) { alo = na; ahi = -1; blo = nb; bhi = -1; }
+            const ShadowCols sc = fl_shadow_cols(c, s, g.SO, SD, xc, yc, g.ya0, g.inv_pa, 0.05f, na);
+            alo = max(alo, sc.alo);
+            ahi = min(ahi, sc.ahi);
+            t = SD / ((xr + g.SO) * g.dz_det);
+#pragma unroll
+            for (int k = 0; k < KZ; k++) {
+                if (iz0 + k < g.nz && alo <= ahi) {
+                    const float zr = zc[k] - sz;
+                    int bh = (int)ceilf((zr + d) * t + (float)nb2);
+                    int bl = (int)floorf((zr - d) * t + (float)nb2);
+                    if (bl < 0) bl = 0;
+                    if (bh > nb - 1) bh = nb - 1;
+                    int sl, sh;
+                    fl_shadow_rows(sc, zr, hz, g.zb0, g.inv_pb, 0.05f, nb, sl, sh);
+                    bl = max(bl, sl);
+                    bh = min(bh, sh);
+                    if (bl <= bh) {
+                        blo[k] = bl;
+                        bhi[k] = bh;
+                        ublo = min(ublo, bl);
+                        ubhi = max(ubhi, bh);
+                    }
+                }
+            }
+            if (ubhi < 0) { alo = na; ahi = -1; }
         }
         // workgroup footprint = union of the voxels' windows
         {
-            int m0 = wave_min(alo), m1 = wave_max(ahi), m2 = wave_min(blo), m3 = wave_max(bhi);
+            int m0 = wave_min(alo), m1 = wave_max(ahi), m2 = wave_min(ublo), m3 = wave_max(ubhi);
             if (lane == 0) {
                 red[j & 1][wv][0] = m0; red[j & 1][wv][1] = m1;
                 red[j & 1][wv][2] = m2; red[j & 1][wv][3] = m3;
@@ -209,48 +182,62 @@ __global__ __launch_bounds__(256) void k_adj_cone_fp(const YT* __restrict__ Y, X
                 q_y[k] = fl_load(y, (size_t)(B0 + kb) * na + (A0 + ka));
             }
             __syncthreads();
-            if (active) {
-                const float p[3] = {x1 - xc, y1 - yc, z1 - zc};
+            if (alo <= ahi) {
                 const float h[3] = {0.5f, 0.5f, hz};
-                for (int ib = blo; ib <= bhi; ib++) {
-                    const int kb = ib - B0;
-                    const float cc = r_c[kb];
-                    for (int ia = alo; ia <= ahi; ia++) {
-                        const int ka = ia - A0;
-                        const float a = c_a[ka], bb = c_b[ka];
-                        float l;
-                        if (a == 0 || bb == 0) {
-                            const float yd = g.ydet[ia];
-                            l = fl_chord_box3(x1, y1, z1, c * g.OD - s * yd, s * g.OD + c * yd,
-                                              g.zdet[ib] + sz, 1.0f, 1.0f, g.dz, xc, yc, zc);
-                        } else {
-                            const float dist = c_nx[ka] * xc + c_ny[ka] * yc - c_off[ka];
-                            if (fl_absf(dist) > c_hs[ka]) continue;
-                            const float t = c_ta[ka] * xc + c_tb[ka] * yc + c_t0[ka];
-                            if (fl_absf(p[2] + cc * t) > hz + fl_absf(cc) * c_hw[ka] + MISS_MARGIN) continue;
-                            const float4 r4 = q_r[kb * nA + ka];
-                            FlRatios r;
-                            r.sba = c_sba[ka]; r.sab = c_sab[ka];
-                            r.sca = r4.x; r.scb = r4.y; r.sac = r4.z; r.sbc = r4.w;
-                            l = fl_chord_box3_r(p, h, fl_absf(a) > fl_absf(bb), r);
+#pragma unroll
+                for (int k = 0; k < KZ; k++) {
+                    const float p[3] = {x1 - xc, y1 - yc, z1 - zc[k]};
+                    for (int ib = blo[k]; ib <= bhi[k]; ib++) {
+                        const int kb = ib - B0;
+                        const float cc = r_c[kb];
+                        for (int ia = alo; ia <= ahi; ia++) {
+                            const int ka = ia - A0;
+                            const float a = c_a[ka], bb = c_b[ka];
+                            float l;
+                            if (a == 0 || bb == 0) {
+                                const float yd = g.ydet[ia];
+                                l = fl_chord_box3(x1, y1, z1, c * g.OD - s * yd, s * g.OD + c * yd,
+                                                  g.zdet[ib] + sz, 1.0f, 1.0f, g.dz, xc, yc, zc[k]);
+                            } else {
+                                const float dist = c_nx[ka] * xc + c_ny[ka] * yc - c_off[ka];
+                                if (fl_absf(dist) > c_hs[ka]) continue;
+                                const float tt = c_ta[ka] * xc + c_tb[ka] * yc + c_t0[ka];
+                                if (fl_absf(p[2] + cc * tt) > hz + fl_absf(cc) * c_hw[ka] + MISS_MARGIN) continue;
+                                const float4 r4 = q_r[kb * nA + ka];
+                                FlRatiosS q;
+                                q.r.sba = c_sba[ka]; q.r.sab = c_sab[ka];
+                                q.r.sca = r4.x; q.r.scb = r4.y; q.r.sac = r4.z; q.r.sbc = r4.w;
+                                q.sg[0] = a > 0 ? 1.0f : -1.0f;
+                                q.sg[1] = bb > 0 ? 1.0f : -1.0f;
+                                q.sg[2] = cc > 0 ? 1.0f : -1.0f;
+                                l = fl_chord_box3_fast(p, h, fl_absf(a) > fl_absf(bb), q);
+                            }
+                            acc[k] = acc[k] + l * q_y[kb * nA + ka];
                         }
-                        acc = acc + l * q_y[kb * nA + ka];
                     }
                 }
             }
-        } else if (active) {
-            for (int ib = blo; ib <= bhi; ib++) {
-                const float z2 = g.zdet[ib] + sz;
-                for (int ia = alo; ia <= ahi; ia++) {
-                    const float yd = g.ydet[ia];
-                    const float l = fl_chord_box3(x1, y1, z1, c * g.OD - s * yd, s * g.OD + c * yd, z2,
-                                                  1.0f, 1.0f, g.dz, xc, yc, zc);
-                    acc = acc + l * fl_load(y, (size_t)ib * na + ia);
+        } else if (alo <= ahi) {
+#pragma unroll
+            for (int k = 0; k < KZ; k++) {
+                for (int ib = blo[k]; ib <= bhi[k]; ib++) {
+                    const float z2 = g.zdet[ib] + sz;
+                    for (int ia = alo; ia <= ahi; ia++) {
+                        const float yd = g.ydet[ia];
+                        const float l = fl_chord_box3(x1, y1, z1, c * g.OD - s * yd, s * g.OD + c * yd, z2,
+                                                      1.0f, 1.0f, g.dz, xc, yc, zc[k]);
+                        acc[k] = acc[k] + l * fl_load(y, (size_t)ib * na + ia);
+                    }
                 }
             }
         }
     }
-    if (active) fl_store(X, (size_t)it * g.nvox + ((size_t)iz * g.ny + iy) * g.nx + ix, acc * g.scale);
+    if (col) {
+#pragma unroll
+        for (int k = 0; k < KZ; k++)
+            if (iz0 + k < g.nz)
+                fl_store(X, (size_t)it * g.nvox + ((size_t)(iz0 + k) * g.ny + iy) * g.nx + ix, acc[k] * g.scale);
+    }
 }
 
 template <typename YT, typename XT>
@@ -265,9 +252,10 @@ static hipError_t launch_adj_fan_t(const void* Y, void* X, const AdjGeom& g, hip
 template <typename YT, typename XT>
 static hipError_t launch_adj_cone_t(const void* Y, void* X, const AdjGeom& g, hipStream_t st)
 {
-    uint32_t nblk = (uint32_t)((g.nx + 7) / 8) * ((g.ny + 7) / 8) * ((g.nz + 3) / 4);
+    constexpr int KZ = 4;
+    uint32_t nblk = (uint32_t)((g.nx + 7) / 8) * ((g.ny + 7) / 8) * ((g.nz + 4 * KZ - 1) / (4 * KZ));
     if (nblk == 0 || g.nt == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_adj_cone_fp<YT, XT>), dim3(nblk, g.nt), dim3(256), 0, st, (const YT*)Y, (XT*)X, g);
+    hipLaunchKernelGGL((k_adj_cone_zc<YT, XT, KZ>), dim3(nblk, g.nt), dim3(256), 0, st, (const YT*)Y, (XT*)X, g);
     return hipGetLastError();
 }
 

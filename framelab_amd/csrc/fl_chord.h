@@ -173,6 +173,63 @@ FL_HD float fl_chord_box3_r(const float* p, const float* h, bool xmajor, const F
     return 0.0f;
 }
 
+// Branch-free form of fl_chord_box3_r for the generic case.  For each face
+// pair the candidate on the face the ray enters through (the -h face when
+// the direction along that axis is positive) and the one it leaves through
+// are formed with the same operations as in fl_face_pair_r: with the sign
+// s of the direction, mu - s*wu equals the lo (s > 0) or hi (s < 0)
+// candidate bit for bit, since x - (-y) and x + y round the same sum.  When
+// exactly one entry and one exit candidate pass, they are the only passing
+// candidates, so they are the two points fl_chord_box3_r takes (in some
+// order; the squared differences do not depend on it) and the length is
+// bit-identical.  With fewer than two passing candidates the length is 0;
+// every other pattern (ray through an edge or corner) takes the ordered
+// routine.  sg = (sign a, sign b, sign c) as +-1.
+struct FlRatiosS {
+    FlRatios r;
+    float sg[3];
+};
+
+FL_HD float fl_chord_box3_fast(const float* p, const float* h, bool xmajor, const FlRatiosS& q)
+{
+    const FlRatios& r = q.r;
+    // x faces: U = y (slope sba), V = z (slope sca)
+    const float mxu = r.sba * (-p[0]) + p[1], wxu = r.sba * h[0] * q.sg[0];
+    const float mxv = r.sca * (-p[0]) + p[2], wxv = r.sca * h[0] * q.sg[0];
+    // y faces: U = x (slope sab), V = z (slope scb)
+    const float myu = r.sab * (-p[1]) + p[0], wyu = r.sab * h[1] * q.sg[1];
+    const float myv = r.scb * (-p[1]) + p[2], wyv = r.scb * h[1] * q.sg[1];
+    // z faces: U = x (slope sac), V = y (slope sbc)
+    const float mzu = r.sac * (-p[2]) + p[0], wzu = r.sac * h[2] * q.sg[2];
+    const float mzv = r.sbc * (-p[2]) + p[1], wzv = r.sbc * h[2] * q.sg[2];
+    // entry (e) and exit (x) candidates
+    const float ex_u = mxu - wxu, ex_v = mxv - wxv, xx_u = mxu + wxu, xx_v = mxv + wxv;
+    const float ey_u = myu - wyu, ey_v = myv - wyv, xy_u = myu + wyu, xy_v = myv + wyv;
+    const float ez_u = mzu - wzu, ez_v = mzv - wzv, xz_u = mzu + wzu, xz_v = mzv + wzv;
+    const bool fex = fl_absf(ex_u) <= h[1] && fl_absf(ex_v) <= h[2];
+    const bool fxx = fl_absf(xx_u) <= h[1] && fl_absf(xx_v) <= h[2];
+    const bool fey = fl_absf(ey_u) <= h[0] && fl_absf(ey_v) <= h[2];
+    const bool fxy = fl_absf(xy_u) <= h[0] && fl_absf(xy_v) <= h[2];
+    const bool fez = fl_absf(ez_u) <= h[0] && fl_absf(ez_v) <= h[1];
+    const bool fxz = fl_absf(xz_u) <= h[0] && fl_absf(xz_v) <= h[1];
+    const int ne = (int)fex + (int)fey + (int)fez, nx = (int)fxx + (int)fxy + (int)fxz;
+    if (ne == 1 && nx == 1) {
+        const float hx = h[0] * q.sg[0], hy = h[1] * q.sg[1], hz = h[2] * q.sg[2];
+        // entry point
+        const float e0 = fex ? -hx : (fey ? ey_u : ez_u);
+        const float e1 = fex ? ex_u : (fey ? -hy : ez_v);
+        const float e2 = fex ? ex_v : (fey ? ey_v : -hz);
+        // exit point
+        const float x0 = fxx ? hx : (fxy ? xy_u : xz_u);
+        const float x1 = fxx ? xx_u : (fxy ? hy : xz_v);
+        const float x2 = fxx ? xx_v : (fxy ? xy_v : hz);
+        const float u = x0 - e0, v = x1 - e1, w = x2 - e2;
+        return sqrtf(u * u + v * v + w * w);
+    }
+    if (ne + nx <= 1) return 0.0f;
+    return fl_chord_box3_r(p, h, xmajor, r);
+}
+
 // Detector cells whose ray can meet a voxel box, from the projection of the
 // box corners (shadow bounding box) widened by `margin` cells.  Rotated
 // frame: xr along the source->centre axis, yr along the detector row; the
@@ -185,6 +242,54 @@ FL_HD float fl_chord_box3_r(const float* p, const float* h, bool xmajor, const F
 struct ShadowWin {
     int alo, ahi, blo, bhi;
 };
+
+// The two halves of fl_shadow_window, for a column of voxels that share
+// (xc, yc): the column range depends on the xy square only; the row range
+// also on the voxel's height zr (relative to the source) and half-height.
+struct ShadowCols {
+    int alo, ahi;
+    float rn, rf;   // SD / depth of the near and far ends of the square
+};
+
+__device__ __forceinline__ ShadowCols fl_shadow_cols(float c, float s, float SO, float SD, float xc, float yc,
+                                                     float ya0, float inv_pa, float margin, int na)
+{
+    const float xr = c * xc + s * yc, yr = -s * xc + c * yc;
+    const float hc = 0.5f * c, hs = 0.5f * s;
+    float umin = FL_INF, umax = -FL_INF;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float ox = (k & 1) ? hc : -hc, oy = (k & 2) ? hs : -hs;
+        const float xk = xr + ox + oy;
+        const float yk = yr + ((k & 1) ? -hs : hs) + ((k & 2) ? hc : -hc);
+        const float u = yk * SD * __builtin_amdgcn_rcpf(xk + SO);
+        umin = fminf(umin, u);
+        umax = fmaxf(umax, u);
+    }
+    const float ex = 0.5f * (fabsf(c) + fabsf(s));
+    ShadowCols w;
+    w.rn = SD * __builtin_amdgcn_rcpf(xr - ex + SO);
+    w.rf = SD * __builtin_amdgcn_rcpf(xr + ex + SO);
+    const float a1 = (umin - ya0) * inv_pa, a2 = (umax - ya0) * inv_pa;
+    w.alo = (int)ceilf(fminf(a1, a2) - margin);
+    w.ahi = (int)floorf(fmaxf(a1, a2) + margin);
+    if (w.alo < 0) w.alo = 0;
+    if (w.ahi > na - 1) w.ahi = na - 1;
+    return w;
+}
+
+__device__ __forceinline__ void fl_shadow_rows(const ShadowCols& sc, float zr, float hz, float zb0, float inv_pb,
+                                               float margin, int nb, int& blo, int& bhi)
+{
+    const float zl = zr - hz, zh = zr + hz;
+    const float vmax = zh > 0 ? zh * sc.rn : zh * sc.rf;
+    const float vmin = zl < 0 ? zl * sc.rn : zl * sc.rf;
+    const float b1 = (vmin - zb0) * inv_pb, b2 = (vmax - zb0) * inv_pb;
+    blo = (int)ceilf(fminf(b1, b2) - margin);
+    bhi = (int)floorf(fmaxf(b1, b2) + margin);
+    if (blo < 0) blo = 0;
+    if (bhi > nb - 1) bhi = nb - 1;
+}
 
 __device__ __forceinline__ ShadowWin fl_shadow_window(float c, float s, float SO, float SD, float xc, float yc,
                                                      float zr, float hz, float ya0, float inv_pa, float zb0,

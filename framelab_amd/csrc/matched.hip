@@ -43,7 +43,7 @@ __device__ __forceinline__ float matched_coef(float inx, float outx, float iny, 
 
 namespace {
 constexpr int MF_A = 48;   // footprint columns in LDS
-constexpr int MF_B = 24;   // footprint rows
+constexpr int MF_B = 32;   // footprint rows
 
 __device__ __forceinline__ int mwave_min(int v)
 {
@@ -57,54 +57,75 @@ __device__ __forceinline__ int mwave_max(int v)
 }
 }  // namespace
 
-// 3D, footprint-staged: per view the workgroup (an 8x8x4 voxel brick) stages
-// in LDS the plane parameters of its nine x planes and nine y planes for
-// each footprint column, of its five z planes for each footprint row (each
-// the forward's IEEE division, done once per workgroup), the ray lengths
-// |d| and the sinogram values; each voxel then visits only the cells of its
-// own shadow.  Views of each frame in order; FP32 accumulation.
-template <typename YT, typename XT>
+// 3D, footprint-staged, z-column form: a workgroup is an 8 x 8 x (4*KZ)
+// voxel brick and each thread owns KZ z-adjacent voxels of one (x, y)
+// column.  Per view the workgroup stages in LDS the plane parameters of its
+// nine x planes and nine y planes for each footprint column, of its 4*KZ+1
+// z planes for each footprint row (each the forward's IEEE division, done
+// once per workgroup), the ray lengths |d| and the sinogram values; each
+// voxel then visits only the cells of its own shadow.  Views of each frame
+// in order, rows outer, columns inner; FP32 accumulation.
+template <typename YT, typename XT, int KZ>
 __global__ __launch_bounds__(256) void k_adj_matched_cone(const YT* __restrict__ Y, XT* __restrict__ X,
                                                          FwdGeom f, AdjGeom g)
 {
+    constexpr int BZ = 4 * KZ;
     __shared__ float t_ax[MF_A][9], t_ay[MF_A][9];
     __shared__ float t_dx[MF_A], t_dy[MF_A];
-    __shared__ float t_az[MF_B][5], t_dz[MF_B];
+    __shared__ float t_az[MF_B][BZ + 1], t_dz[MF_B];
     __shared__ float t_L[MF_A * MF_B], t_y[MF_A * MF_B];
     __shared__ int red[2][4][4];
 
-    const int bx = (f.nx + 7) / 8, by = (f.ny + 7) / 8, bz = (f.nz + 3) / 4;
+    const int bx = (f.nx + 7) / 8, by = (f.ny + 7) / 8, bz = (f.nz + BZ - 1) / BZ;
     const uint32_t nblk = (uint32_t)bx * by * bz;
     const uint32_t b = fl_xcd_remap(blockIdx.x, nblk);
     const int it = blockIdx.y;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int ix0 = (int)(b % bx) * 8, iy0 = (int)((b / bx) % by) * 8, iz0 = (int)(b / ((uint32_t)bx * by)) * 4;
-    const int lx = tid & 7, ly = (tid >> 3) & 7, lz = tid >> 6;
-    const int ix = ix0 + lx, iy = iy0 + ly, iz = iz0 + lz;
-    const bool active = ix < f.nx && iy < f.ny && iz < f.nz;
+    const int ix0 = (int)(b % bx) * 8, iy0 = (int)((b / bx) % by) * 8, iz0 = (int)(b / ((uint32_t)bx * by)) * BZ;
+    const int lx = tid & 7, ly = (tid >> 3) & 7, lz0 = wv * KZ;
+    const int ix = ix0 + lx, iy = iy0 + ly;
+    const bool col = ix < f.nx && iy < f.ny;
 
     const int na = f.na, nb = f.nb;
     const float SD = f.SO + f.OD;
     const float xc = (float)ix + 0.5f - (float)(f.nx / 2);
     const float yc = (float)iy + 0.5f - (float)(f.ny / 2);
-    const float zc = ((float)iz + 0.5f - (float)(f.nz / 2)) * f.dz0;
     const float hz = 0.5f * f.dz0;
     const bool dz_one = f.dz0 == 1.0f;
-    float acc = 0.0f;
+    float acc[KZ];
+#pragma unroll
+    for (int k = 0; k < KZ; k++) acc[k] = 0.0f;
     const int nvf = g.Nv[it];
     for (int j = 0; j < nvf; j++) {
         const int iv = g.id_Y[it * g.tmp_size + j];
         const YT* __restrict__ y = Y + (size_t)iv * g.nd;
         const float c = f.cs[iv], s = f.sn[iv];
         const float sz = f.sdz[iv];
-        int alo = na, ahi = -1, blo = nb, bhi = -1;
-        if (active) {
-            const ShadowWin w = fl_shadow_window(c, s, f.SO, SD, xc, yc, zc - sz, hz, f.ya0, f.inv_pa, f.zb0,
-                                                 f.inv_pb, 0.05f, na, nb);
-            if (w.alo <= w.ahi && w.blo <= w.bhi) { alo = w.alo; ahi = w.ahi; blo = w.blo; bhi = w.bhi; }
+        int alo = na, ahi = -1, ublo = nb, ubhi = -1;
+        int blo[KZ], bhi[KZ];
+#pragma unroll
+        for (int k = 0; k < KZ; k++) { blo[k] = nb; bhi[k] = -1; }
+        if (col) {
+            const ShadowCols sc = fl_shadow_cols(c, s, f.SO, SD, xc, yc, f.ya0, f.inv_pa, 0.05f, na);
+            alo = sc.alo;
+            ahi = sc.ahi;
+#pragma unroll
+            for (int k = 0; k < KZ; k++) {
+                const int iz = iz0 + lz0 + k;
+                if (iz < f.nz && alo <= ahi) {
+                    const float zc = ((float)iz + 0.5f - (float)(f.nz / 2)) * f.dz0;
+                    int bl, bh;
+                    fl_shadow_rows(sc, zc - sz, hz, f.zb0, f.inv_pb, 0.05f, nb, bl, bh);
+                    if (bl <= bh) {
+                        blo[k] = bl; bhi[k] = bh;
+                        ublo = min(ublo, bl); ubhi = max(ubhi, bh);
+                    }
+                }
+            }
+            if (ubhi < 0) { alo = na; ahi = -1; }
         }
         {
-            int m0 = mwave_min(alo), m1 = mwave_max(ahi), m2 = mwave_min(blo), m3 = mwave_max(bhi);
+            int m0 = mwave_min(alo), m1 = mwave_max(ahi), m2 = mwave_min(ublo), m3 = mwave_max(ubhi);
             if (lane == 0) {
                 red[j & 1][wv][0] = m0; red[j & 1][wv][1] = m1;
                 red[j & 1][wv][2] = m2; red[j & 1][wv][3] = m3;
@@ -131,8 +152,8 @@ __global__ __launch_bounds__(256) void k_adj_matched_cone(const YT* __restrict__
                 t_ay[ka][q] = ((float)(iy0 + q - f.ny / 2) - y1) / dy;
                 if (q == 0) { t_dx[ka] = dx; t_dy[ka] = dy; }
             }
-            for (int k = tid; k < nB * 5; k += 256) {
-                const int kb = k / 5, q = k % 5;
+            for (int k = tid; k < nB * (BZ + 1); k += 256) {
+                const int kb = k / (BZ + 1), q = k % (BZ + 1);
                 const float dz = (f.zdet[B0 + kb] + sz) - z1;
                 t_az[kb][q] = ((float)(iz0 + q - f.nz / 2) * f.dz0 - z1) / dz;
                 if (q == 0) t_dz[kb] = dz;
@@ -146,56 +167,70 @@ __global__ __launch_bounds__(256) void k_adj_matched_cone(const YT* __restrict__
                 t_y[k] = fl_load(y, (size_t)(B0 + kb) * na + (A0 + ka));
             }
             __syncthreads();
-            if (active && alo <= ahi) {
-                for (int ib = blo; ib <= bhi; ib++) {
-                    const int kb = ib - B0;
-                    const float dz = t_dz[kb];
-                    SidAxis az;
-                    sid_axis_init(az, z1, dz, f.dz0, f.nz);
-                    const float z_lo = t_az[kb][lz], z_hi = t_az[kb][lz + 1];
-                    const float inz = az.on ? (dz > 0 ? z_lo : z_hi) : -FL_INF;
-                    const float outz = az.on ? (dz > 0 ? z_hi : z_lo) : FL_INF;
-                    for (int ia = alo; ia <= ahi; ia++) {
-                        const int ka = ia - A0;
-                        const float dx = t_dx[ka], dy = t_dy[ka];
-                        SidAxis ax, ay;
-                        sid_axis_init(ax, x1, dx, 1.0f, f.nx);
-                        sid_axis_init(ay, y1, dy, 1.0f, f.ny);
-                        const float x_lo = t_ax[ka][lx], x_hi = t_ax[ka][lx + 1];
-                        const float y_lo = t_ay[ka][ly], y_hi = t_ay[ka][ly + 1];
-                        const float inx = ax.on ? (dx > 0 ? x_lo : x_hi) : -FL_INF;
-                        const float outx = ax.on ? (dx > 0 ? x_hi : x_lo) : FL_INF;
-                        const float iny = ay.on ? (dy > 0 ? y_lo : y_hi) : -FL_INF;
-                        const float outy = ay.on ? (dy > 0 ? y_hi : y_lo) : FL_INF;
-                        const int k = kb * nA + ka;
-                        const float Si = matched_coef(inx, outx, iny, outy, inz, outz, ax, ay, az, ix, iy, iz,
-                                                      dz_one, t_L[k]);
-                        if (Si != 0.0f) acc = acc + Si * t_y[k];
+            if (alo <= ahi) {
+#pragma unroll
+                for (int k = 0; k < KZ; k++) {
+                    const int lz = lz0 + k, iz = iz0 + lz;
+                    for (int ib = blo[k]; ib <= bhi[k]; ib++) {
+                        const int kb = ib - B0;
+                        const float dz = t_dz[kb];
+                        SidAxis az;
+                        sid_axis_init(az, z1, dz, f.dz0, f.nz);
+                        const float z_lo = t_az[kb][lz], z_hi = t_az[kb][lz + 1];
+                        const float inz = az.on ? (dz > 0 ? z_lo : z_hi) : -FL_INF;
+                        const float outz = az.on ? (dz > 0 ? z_hi : z_lo) : FL_INF;
+                        for (int ia = alo; ia <= ahi; ia++) {
+                            const int ka = ia - A0;
+                            const float dx = t_dx[ka], dy = t_dy[ka];
+                            SidAxis ax, ay;
+                            sid_axis_init(ax, x1, dx, 1.0f, f.nx);
+                            sid_axis_init(ay, y1, dy, 1.0f, f.ny);
+                            const float x_lo = t_ax[ka][lx], x_hi = t_ax[ka][lx + 1];
+                            const float y_lo = t_ay[ka][ly], y_hi = t_ay[ka][ly + 1];
+                            const float inx = ax.on ? (dx > 0 ? x_lo : x_hi) : -FL_INF;
+                            const float outx = ax.on ? (dx > 0 ? x_hi : x_lo) : FL_INF;
+                            const float iny = ay.on ? (dy > 0 ? y_lo : y_hi) : -FL_INF;
+                            const float outy = ay.on ? (dy > 0 ? y_hi : y_lo) : FL_INF;
+                            const int q = kb * nA + ka;
+                            const float Si = matched_coef(inx, outx, iny, outy, inz, outz, ax, ay, az, ix, iy, iz,
+                                                          dz_one, t_L[q]);
+                            if (Si != 0.0f) acc[k] = acc[k] + Si * t_y[q];
+                        }
                     }
                 }
             }
-        } else if (active && alo <= ahi) {
-            for (int ib = blo; ib <= bhi; ib++) {
-                const float dz = (f.zdet[ib] + sz) - z1;
-                SidAxis az;
-                sid_axis_init(az, z1, dz, f.dz0, f.nz);
-                const FlAxisIv sv = fl_slab(az, iz);
-                for (int ia = alo; ia <= ahi; ia++) {
-                    const float yd = f.ydet[ia];
-                    const float dx = (c * f.OD - s * yd) - x1, dy = (s * f.OD + c * yd) - y1;
-                    SidAxis ax, ay;
-                    sid_axis_init(ax, x1, dx, 1.0f, f.nx);
-                    sid_axis_init(ay, y1, dy, 1.0f, f.ny);
-                    const FlAxisIv sx = fl_slab(ax, ix), sy = fl_slab(ay, iy);
-                    const float L = sqrtf(dx * dx + dy * dy + dz * dz);
-                    const float Si = matched_coef(sx.in, sx.out, sy.in, sy.out, sv.in, sv.out, ax, ay, az, ix, iy,
-                                                  iz, dz_one, L);
-                    if (Si != 0.0f) acc = acc + Si * fl_load(y, (size_t)ib * na + ia);
+        } else if (alo <= ahi) {
+#pragma unroll
+            for (int k = 0; k < KZ; k++) {
+                const int iz = iz0 + lz0 + k;
+                for (int ib = blo[k]; ib <= bhi[k]; ib++) {
+                    const float dz = (f.zdet[ib] + sz) - z1;
+                    SidAxis az;
+                    sid_axis_init(az, z1, dz, f.dz0, f.nz);
+                    const FlAxisIv sv = fl_slab(az, iz);
+                    for (int ia = alo; ia <= ahi; ia++) {
+                        const float yd = f.ydet[ia];
+                        const float dx = (c * f.OD - s * yd) - x1, dy = (s * f.OD + c * yd) - y1;
+                        SidAxis ax, ay;
+                        sid_axis_init(ax, x1, dx, 1.0f, f.nx);
+                        sid_axis_init(ay, y1, dy, 1.0f, f.ny);
+                        const FlAxisIv sx = fl_slab(ax, ix), sy = fl_slab(ay, iy);
+                        const float L = sqrtf(dx * dx + dy * dy + dz * dz);
+                        const float Si = matched_coef(sx.in, sx.out, sy.in, sy.out, sv.in, sv.out, ax, ay, az, ix,
+                                                      iy, iz, dz_one, L);
+                        if (Si != 0.0f) acc[k] = acc[k] + Si * fl_load(y, (size_t)ib * na + ia);
+                    }
                 }
             }
         }
     }
-    if (active) fl_store(X, (size_t)it * f.nvox + ((size_t)iz * f.ny + iy) * f.nx + ix, acc * f.scale);
+    if (col) {
+#pragma unroll
+        for (int k = 0; k < KZ; k++) {
+            const int iz = iz0 + lz0 + k;
+            if (iz < f.nz) fl_store(X, (size_t)it * f.nvox + ((size_t)iz * f.ny + iy) * f.nx + ix, acc[k] * f.scale);
+        }
+    }
 }
 
 template <typename YT, typename XT>
@@ -214,8 +249,9 @@ static hipError_t launch_matched_t(const void* Y, void* X, const fl_plan_s* p, h
 {
     const FwdGeom& f = p->fg;
     if (p->cone) {
-        uint32_t nblk = (uint32_t)((f.nx + 7) / 8) * ((f.ny + 7) / 8) * ((f.nz + 3) / 4);
-        hipLaunchKernelGGL((k_adj_matched_cone<YT, XT>), dim3(nblk, p->nt), dim3(256), 0, st, (const YT*)Y,
+        constexpr int KZ = 4;
+        uint32_t nblk = (uint32_t)((f.nx + 7) / 8) * ((f.ny + 7) / 8) * ((f.nz + 4 * KZ - 1) / (4 * KZ));
+        hipLaunchKernelGGL((k_adj_matched_cone<YT, XT, KZ>), dim3(nblk, p->nt), dim3(256), 0, st, (const YT*)Y,
                            (XT*)X, f, p->ag);
     } else {
         dim3 grid((f.nx + 15) / 16, (f.ny + 15) / 16, p->nt);
