@@ -36,7 +36,7 @@ __global__ __launch_bounds__(256) void k_adj_fan(const YT* __restrict__ Y, XT* _
 // that starts at +0 unchanged), so the result is bit-identical to the
 // reference kernel.
 namespace {
-constexpr int FP_A = 48;    // footprint columns held in LDS
+constexpr int FP_A = 40;    // footprint columns held in LDS
 constexpr int FP_B = 32;    // footprint rows
 constexpr float MISS_MARGIN = 0.02f;
 
@@ -59,8 +59,10 @@ __device__ __forceinline__ int wave_max(int v)
 // the order of accumulation are those of the reference kernel
 // (Atx_cone_mf_gpu_new.cu:58-86: views in order, rows outer, columns
 // inner), restricted to the cells whose chord can be non-zero.
+// Five workgroups per CU (the LDS tables take 27.7 KB each; the register
+// budget that allows five is the measured optimum at 256^3).
 template <typename YT, typename XT, int KZ>
-__global__ __launch_bounds__(256) void k_adj_cone_zc(const YT* __restrict__ Y, XT* __restrict__ X, AdjGeom g)
+__global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y, XT* __restrict__ X, AdjGeom g)
 {
     __shared__ float c_a[FP_A], c_b[FP_A], c_sba[FP_A], c_sab[FP_A];
     __shared__ float c_nx[FP_A], c_ny[FP_A], c_off[FP_A], c_hs[FP_A];
