@@ -18,7 +18,7 @@ FL_ERR_ALLOC = -3
 FL_ERR_UNSUPPORTED = -4
 
 FL_F32, FL_BF16, FL_F16 = 0, 1, 2
-FL_ADJ_EXACT, FL_ADJ_MATCHED = 0, 1
+FL_ADJ_EXACT, FL_ADJ_MATCHED, FL_ADJ_FAN_FB = 0, 1, 2
 
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int)
@@ -37,6 +37,7 @@ class FlGeometry(C.Structure):
         ("id_X", _ip), ("id_Y", _ip), ("Nv", _ip),
         ("tmp_size", C.c_int),
         ("slot_view", _ip),
+        ("cos_det2", _fp), ("sin_det2", _fp),
     ]
 
 
@@ -77,6 +78,12 @@ SIGNATURES = {
                                 C.POINTER(C.c_longlong), C.POINTER(C.c_int)]),
     "fl_forward": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_int, _vp]),
     "fl_forward_gao_fan": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_int, _vp]),
+    "fl_forward_fan_fb": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "Ax_fan_mf_gpu_new_fb": (None, [_fp, _fp, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int,
+                                    C.c_int, _fp, C.c_int, _fp, _ip, C.c_int, C.c_float]),
+    "Atx_fan_mf_gpu_new_fb": (None, [_fp, _fp, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, _fp,
+                                     C.c_int, _fp, C.c_float, C.c_int, _ip, _ip, C.c_int, C.c_int,
+                                     _fp, _fp, _fp, _fp]),
     "fl_adjoint": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_int, C.c_int, _vp]),
     "fl_siddon_trace": (C.c_int, [_vp, C.c_int, _ip, C.c_int, _ip, _fp, _ip]),
     "fl_framelet_num_bands": (C.c_int, [C.c_int, C.c_int]),

@@ -71,7 +71,8 @@ static int upload(T** dst, const T* src, size_t n, const char* what)
 static void plan_free_arrays(fl_plan_s* p)
 {
     void* ptrs[] = {p->d_cs, p->d_sn, p->d_sdz, p->d_ydet, p->d_zdet, p->d_cphi, p->d_sphi,
-                    p->d_cdet, p->d_sdet, p->d_slot_view, p->d_view_frame, p->d_idY, p->d_Nv, p->d_bricks};
+                    p->d_cdet, p->d_sdet, p->d_slot_view, p->d_view_frame, p->d_idY, p->d_Nv, p->d_bricks,
+                    p->d_fb_px, p->d_fb_py, p->d_fb_pd, p->d_cdet2, p->d_sdet2};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
 }
@@ -193,6 +194,14 @@ extern "C" int fl_plan_create(fl_plan* out, const fl_geometry* g)
         delete p;
         return rc;
     }
+    if (!cone && g->cos_det2 && g->sin_det2 &&
+        ((rc = upload(&p->d_cdet2, g->cos_det2, (size_t)g->na + 1, "cos_det2")) ||
+         (rc = upload(&p->d_sdet2, g->sin_det2, (size_t)g->na + 1, "sin_det2")))) {
+        plan_free_arrays(p);
+        delete p;
+        return rc;
+    }
+    if (!cone) p->h_ydet.assign(g->y_det, g->y_det + g->na);
     if (cone) {
         // the forward gathers from a bricked copy of its input (8x8x1 bricks,
         // padded to whole bricks; 4-byte elements cover every storage type)
@@ -358,6 +367,46 @@ extern "C" int fl_forward_gao_fan(fl_plan p, const void* d_x, int xt, void* d_y,
     return fl_ok();
 }
 
+// Strip lines of the finite-width fan forward per (view, cell, edge), with
+// the reference's float trig (Ax_fan_mf_cpu_new_fb.cpp:38-47): edge
+// u = y_det -+ dy_det/2, angle atan2f(u, SD), Px = -sinf(angle + phi),
+// Py = cosf(angle + phi), Pd = SO*sinf(angle).
+static int ensure_fb_tables(fl_plan_s* p)
+{
+    if (p->d_fb_px) return FL_OK;
+    const int nv = p->nv, nd = (int)p->ndet;
+    const float SO = p->fg.SO, SD = p->fg.SO + p->fg.OD, dy = p->ag.dy_det;
+    std::vector<float> px((size_t)nv * nd * 2), py((size_t)nv * nd * 2), pd((size_t)nd * 2);
+    for (int id = 0; id < nd; id++)
+        for (int e = 0; e < 2; e++) {
+            const float u = e == 0 ? p->h_ydet[id] - dy / 2 : p->h_ydet[id] + dy / 2;
+            const float ang = atan2f(u, SD);
+            pd[(size_t)id * 2 + e] = SO * sinf(ang);
+            for (int iv = 0; iv < nv; iv++) {
+                const float a = ang + p->h_sd_phi[iv];
+                px[((size_t)iv * nd + id) * 2 + e] = -sinf(a);
+                py[((size_t)iv * nd + id) * 2 + e] = cosf(a);
+            }
+        }
+    int rc;
+    if ((rc = upload(&p->d_fb_px, px.data(), px.size(), "strip Px")) ||
+        (rc = upload(&p->d_fb_py, py.data(), py.size(), "strip Py")) ||
+        (rc = upload(&p->d_fb_pd, pd.data(), pd.size(), "strip Pd")))
+        return rc;
+    return FL_OK;
+}
+
+extern "C" int fl_forward_fan_fb(fl_plan p, const float* d_x, float* d_y, void* stream)
+{
+    if (!p || !d_x || !d_y) return fl_fail(FL_ERR_INVALID, "null plan or buffer");
+    if (p->cone) return fl_fail(FL_ERR_INVALID, "the finite-width beam is a fan projector");
+    int rc = ensure_fb_tables(p);
+    if (rc) return rc;
+    hipError_t e = fl_launch_fwd_fan_fb(d_x, d_y, p, (hipStream_t)stream);
+    if (e != hipSuccess) return fl_hip_fail(e, "finite-beam forward launch");
+    return fl_ok();
+}
+
 extern "C" int fl_adjoint(fl_plan p, const void* d_y, int yt, void* d_x, int xt, int kind, void* stream)
 {
     if (!p || !d_x || !d_y) return fl_fail(FL_ERR_INVALID, "null plan or buffer");
@@ -369,6 +418,11 @@ extern "C" int fl_adjoint(fl_plan p, const void* d_y, int yt, void* d_x, int xt,
                     : fl_launch_adj_fan(d_y, yt, d_x, xt, p->ag, st);
     } else if (kind == FL_ADJ_MATCHED) {
         e = fl_launch_adj_matched(d_y, yt, d_x, xt, p, st);
+    } else if (kind == FL_ADJ_FAN_FB) {
+        if (p->cone) return fl_fail(FL_ERR_INVALID, "the finite-width beam is a fan projector");
+        if (!p->d_cdet2) return fl_fail(FL_ERR_INVALID, "plan has no cos_det2/sin_det2 edge tables");
+        if (yt != FL_F32 || xt != FL_F32) return fl_fail(FL_ERR_UNSUPPORTED, "finite-width beam is FP32");
+        e = fl_launch_adj_fan_fb((const float*)d_y, (float*)d_x, p, st);
     } else {
         return fl_fail(FL_ERR_INVALID, "unknown adjoint kind %d", kind);
     }
@@ -455,7 +509,7 @@ void frames_to_views(int nt, int nv, const int* id_Y, const int* Nv, int tmp, st
         }
 }
 
-int run_forward_host(const fl_geometry& g, const float* X, float* y, float* yNorm, bool gao)
+int run_forward_host(const fl_geometry& g, const float* X, float* y, float* yNorm, bool gao, bool fb = false)
 {
     PlanGuard pg;
     int rc = fl_plan_create(&pg.p, &g);
@@ -465,8 +519,9 @@ int run_forward_host(const fl_geometry& g, const float* X, float* y, float* yNor
     if ((rc = dx.alloc(nvox * 4, "X")) || (rc = dy.alloc(ny * 4, "y"))) return rc;
     hipError_t e = hipMemcpy(dx.p, X, nvox * 4, hipMemcpyHostToDevice);
     if (e != hipSuccess) return fl_hip_fail(e, "copy X");
-    rc = gao ? fl_forward_gao_fan(pg.p, dx.p, FL_F32, dy.p, FL_F32, nullptr)
-             : fl_forward(pg.p, dx.p, FL_F32, dy.p, FL_F32, nullptr);
+    rc = fb    ? fl_forward_fan_fb(pg.p, (const float*)dx.p, (float*)dy.p, nullptr)
+         : gao ? fl_forward_gao_fan(pg.p, dx.p, FL_F32, dy.p, FL_F32, nullptr)
+               : fl_forward(pg.p, dx.p, FL_F32, dy.p, FL_F32, nullptr);
     if (rc) return rc;
     if (yNorm) {
         if ((rc = dn.alloc(ny * 4, "yNorm"))) return rc;
@@ -479,7 +534,7 @@ int run_forward_host(const fl_geometry& g, const float* X, float* y, float* yNor
     return fl_ok();
 }
 
-int run_adjoint_host(const fl_geometry& g, const float* Y, float* X)
+int run_adjoint_host(const fl_geometry& g, const float* Y, float* X, int kind = FL_ADJ_EXACT)
 {
     PlanGuard pg;
     int rc = fl_plan_create(&pg.p, &g);
@@ -489,7 +544,7 @@ int run_adjoint_host(const fl_geometry& g, const float* Y, float* X)
     if ((rc = dx.alloc(nvox * 4, "X")) || (rc = dy.alloc(ny * 4, "y"))) return rc;
     hipError_t e = hipMemcpy(dy.p, Y, ny * 4, hipMemcpyHostToDevice);
     if (e != hipSuccess) return fl_hip_fail(e, "copy y");
-    if ((rc = fl_adjoint(pg.p, dy.p, FL_F32, dx.p, FL_F32, FL_ADJ_EXACT, nullptr))) return rc;
+    if ((rc = fl_adjoint(pg.p, dy.p, FL_F32, dx.p, FL_F32, kind, nullptr))) return rc;
     e = hipMemcpy(X, dx.p, nvox * 4, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return fl_hip_fail(e, "copy X");
     return fl_ok();
@@ -533,6 +588,35 @@ extern "C" void Atx_fan_mf_gpu_new(float* x, float* Y, float SO, float OD, float
         g.id_X = idX.data();
     }
     run_adjoint_host(g, Y, x);
+}
+
+extern "C" void Ax_fan_mf_gpu_new_fb(float* X, float* y, float SO, float OD, float scale, int nx, int ny,
+                                     int nv, float* sd_phi, int nd, float* y_det, int* id_X, int nt,
+                                     float dy_det)
+{
+    fl_geometry g = {};
+    g.nx = nx; g.ny = ny; g.nz = 0; g.nt = nt; g.nv = nv; g.na = nd; g.nb = 1;
+    g.SO = SO; g.OD = OD; g.scale = scale; g.dy_det = dy_det;
+    g.sd_phi = sd_phi; g.y_det = y_det; g.id_X = id_X;
+    run_forward_host(g, X, y, nullptr, false, true);
+}
+
+extern "C" void Atx_fan_mf_gpu_new_fb(float* x, float* Y, float SO, float OD, float scale, int nx, int ny,
+                                      float* sd_phi, int nd, float* y_det, float dy_det, int nt, int* id_Y,
+                                      int* Nv, int tmp_size, int nv, float* cos_phi, float* sin_phi,
+                                      float* cos_det, float* sin_det)
+{
+    fl_geometry g = {};
+    g.nx = nx; g.ny = ny; g.nz = 0; g.nt = nt; g.nv = nv; g.na = nd; g.nb = 1;
+    g.SO = SO; g.OD = OD; g.scale = scale; g.dy_det = dy_det;
+    g.sd_phi = sd_phi; g.y_det = y_det; g.cos_phi = cos_phi; g.sin_phi = sin_phi;
+    g.cos_det2 = cos_det; g.sin_det2 = sin_det; g.id_Y = id_Y; g.Nv = Nv; g.tmp_size = tmp_size;
+    std::vector<int> idX, slots;
+    if (id_Y && Nv) {
+        frames_to_views(nt, nv, id_Y, Nv, tmp_size, idX, slots);
+        g.id_X = idX.data();
+    }
+    run_adjoint_host(g, Y, x, FL_ADJ_FAN_FB);
 }
 
 static void cone_fwd_dropin(float* X, float* y, float* yNorm, float* sd_phi, float* sd_z, float* y_det,

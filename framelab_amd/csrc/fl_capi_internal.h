@@ -20,6 +20,11 @@ struct fl_plan_s {
     float *d_cphi = nullptr, *d_sphi = nullptr, *d_cdet = nullptr, *d_sdet = nullptr;
     int *d_slot_view = nullptr, *d_view_frame = nullptr, *d_idY = nullptr, *d_Nv = nullptr;
     void* d_bricks = nullptr;   // cone: bricked copy of the forward input (nt frames, 8x8 bricks, 4-byte)
+    // fan, finite-width beam: strip lines per (view, cell, edge) and cell
+    // edge tables (built on first use / uploaded when given)
+    float *d_fb_px = nullptr, *d_fb_py = nullptr, *d_fb_pd = nullptr;
+    float *d_cdet2 = nullptr, *d_sdet2 = nullptr;
+    std::vector<float> h_ydet;
     // lazily allocated workspace (solvers, gradients)
     void* ws = nullptr;
     size_t ws_bytes = 0;
@@ -41,3 +46,5 @@ hipError_t fl_launch_ynorm(const float* y, float* yn, const FwdGeom& g, hipStrea
 hipError_t fl_launch_adj_fan(const void* Y, int yt, void* X, int xt, const AdjGeom& g, hipStream_t st);
 hipError_t fl_launch_adj_cone(const void* Y, int yt, void* X, int xt, const AdjGeom& g, hipStream_t st);
 hipError_t fl_launch_adj_matched(const void* Y, int yt, void* X, int xt, const fl_plan_s* p, hipStream_t st);
+hipError_t fl_launch_fwd_fan_fb(const float* X, float* Y, const fl_plan_s* p, hipStream_t st);
+hipError_t fl_launch_adj_fan_fb(const float* Y, float* X, const fl_plan_s* p, hipStream_t st);

@@ -53,6 +53,10 @@ class FanPara:
     tmp_size: int
     version: int = 1
     GPU: int = 1
+    # detector-cell edge angles for the finite-width beam (version 2): edge e
+    # of nd+1 bounds cells e-1 and e
+    cos_det2: np.ndarray | None = None
+    sin_det2: np.ndarray | None = None
 
     @property
     def nd(self) -> int:
@@ -131,11 +135,19 @@ def fan_para(nd: int, nv: int, N=(256, 256), L=(2.0, 2.0), SO=5.0, OD=5.0, Ly=6.
     cos_phi, sin_phi = _cos_sin_single(sd_phi)
     ang = np.arctan2(y_det.astype(np.float64), np.float64(f32(SOs + ODs))).astype(f32)
     cos_det, sin_det = _cos_sin_single(ang)
+    # cell edges y_det[e] - dy/2 (and the last cell's upper edge), the
+    # strips of the finite-width forward (Ax_fan_mf_cpu_new_fb.cpp:38-47)
+    dyp = f32(dy_det / scale)
+    edges = np.concatenate([(y_det - f32(dyp / f32(2))).astype(f32),
+                            [f32(y_det[-1] + f32(dyp / f32(2)))]]).astype(f32)
+    ang2 = np.arctan2(edges.astype(np.float64), np.float64(f32(SOs + ODs))).astype(f32)
+    cos_det2, sin_det2 = _cos_sin_single(ang2)
     id_X, id_Y, Nv, tmp = _frame_tables(nv)
     return FanPara(SO=f32(SOs), OD=f32(ODs), scale=scale, nx=nx, ny=ny, nt=1,
-                   dy_det=f32(dy_det / scale), y_os=f32(0.0), sd_phi=sd_phi, y_det=y_det,
+                   dy_det=dyp, y_os=f32(0.0), sd_phi=sd_phi, y_det=y_det,
                    cos_phi=cos_phi, sin_phi=sin_phi, cos_det=cos_det, sin_det=sin_det,
-                   id_X=id_X, id_Y=id_Y, Nv=Nv, nv=nv, tmp_size=tmp)
+                   id_X=id_X, id_Y=id_Y, Nv=Nv, nv=nv, tmp_size=tmp,
+                   cos_det2=cos_det2, sin_det2=sin_det2)
 
 
 def cone_para(na: int, nb: int, nv: int, N=(256, 256, 256), L=(2.0, 2.0, 2.0),

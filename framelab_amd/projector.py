@@ -23,6 +23,7 @@ _DT = {torch.float32: _lib.FL_F32, torch.bfloat16: _lib.FL_BF16, torch.float16: 
 
 ADJ_EXACT = _lib.FL_ADJ_EXACT
 ADJ_MATCHED = _lib.FL_ADJ_MATCHED
+ADJ_FAN_FB = _lib.FL_ADJ_FAN_FB
 
 
 def _fptr(a: np.ndarray):
@@ -61,6 +62,8 @@ def geometry_struct(p, keep: list) -> _lib.FlGeometry:
         g.nz, g.nv, g.na, g.nb = 0, p.nv, p.nd, 1
         g.dz, g.dz_det = 1.0, 1.0
         g.cos_det, g.sin_det = f32(p.cos_det), f32(p.sin_det)
+        if getattr(p, "cos_det2", None) is not None:
+            g.cos_det2, g.sin_det2 = f32(p.cos_det2), f32(p.sin_det2)
     return g
 
 
@@ -121,11 +124,20 @@ class Projector:
     # -- operators -----------------------------------------------------------
     def forward(self, x: torch.Tensor, out: torch.Tensor | None = None, dtype=None,
                 method: str = "siddon") -> torch.Tensor:
-        """y = A x (Siddon; method='gao' selects the fan column walk)."""
+        """y = A x (Siddon; method='gao' selects the fan column walk, 'fb' the fan
+        finite-width-beam strip areas, FP32 only)."""
         self._check_tensor(x, self.nvox * self.nt, "x")
         if out is None:
             out = torch.empty(self.sino_shape, dtype=dtype or x.dtype, device=x.device)
         self._check_tensor(out, self.ndet * self.nv, "out")
+        if method == "fb":
+            if x.dtype != torch.float32 or out.dtype != torch.float32:
+                raise ValueError("the finite-width beam is FP32")
+            _lib.check(_lib.lib().fl_forward_fan_fb(self._h, C.c_void_p(x.data_ptr()),
+                                                    C.c_void_p(out.data_ptr()), _stream()))
+            return out
+        if method not in ("siddon", "gao"):
+            raise ValueError(f"unknown forward method {method!r}")
         fn = _lib.lib().fl_forward if method == "siddon" else _lib.lib().fl_forward_gao_fan
         _lib.check(fn(self._h, C.c_void_p(x.data_ptr()), _DT[x.dtype], C.c_void_p(out.data_ptr()),
                       _DT[out.dtype], _stream()))
@@ -133,7 +145,8 @@ class Projector:
 
     def adjoint(self, y: torch.Tensor, out: torch.Tensor | None = None, dtype=None,
                 kind: int = ADJ_EXACT) -> torch.Tensor:
-        """x = A^T y; kind ADJ_EXACT (reference chords) or ADJ_MATCHED (Siddon transpose)."""
+        """x = A^T y; kind ADJ_EXACT (reference chords), ADJ_MATCHED (Siddon transpose) or
+        ADJ_FAN_FB (fan finite-width-beam strip areas, FP32)."""
         self._check_tensor(y, self.ndet * self.nv, "y")
         if out is None:
             out = torch.empty(self.vol_shape, dtype=dtype or y.dtype, device=y.device)

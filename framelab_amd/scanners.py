@@ -17,7 +17,7 @@ import torch
 
 from . import geometry as G
 from .datatypes import CTData, ObjectData
-from .projector import ADJ_EXACT, Projector
+from .projector import ADJ_EXACT, ADJ_FAN_FB, Projector
 
 
 class _Scanner:
@@ -41,7 +41,7 @@ class FanBeamScanner(_Scanner):
         self.nd, self.nv = nd, nv
         self.SO, self.OD, self.Ly = SO, OD, Ly
         self.y_os = 0.0
-        self.version = version       # 1: Gao column walk (class default), 0: Siddon
+        self.version = version       # 1: Gao column walk (class default), 0: Siddon, 2: finite-width beam
         self.para = None
         self._pkey = None
 
@@ -53,13 +53,14 @@ class FanBeamScanner(_Scanner):
     def apply(self, obj: ObjectData) -> CTData:
         P = self.setPara(obj)
         x = obj.dataArray.to("cuda").float().contiguous()
-        y = P.forward(x, method="gao" if self.version == 1 else "siddon")
+        y = P.forward(x, method={0: "siddon", 1: "gao", 2: "fb"}[self.version])
         return CTData(self, y, None, obj.L)
 
     def applyAdjoint(self, data: CTData) -> ObjectData:
         if self._pkey is None:
             raise RuntimeError("applyAdjoint before apply: the scanner has no geometry yet")
-        x = self._proj.adjoint(data.dataArray.contiguous(), kind=self.adjoint_kind)
+        kind = ADJ_FAN_FB if self.version == 2 else self.adjoint_kind
+        x = self._proj.adjoint(data.dataArray.contiguous(), kind=kind)
         return ObjectData(2, x, self._pkey[1])
 
 

@@ -60,6 +60,20 @@ void Atx_fan_mf_gpu_new(float *x, float *Y, float SO, float OD, float scale, int
                         int *id_Y, int *Nv, int tmp_size, int nv, float *cos_phi, float *sin_phi,
                         float *cos_det, float *sin_det);
 
+/* 2D fan forward, finite-width beam (version 2).  Replaces
+ * Ax_fan_mf_gpu_new_fb (matlab/mex/src/Ax_fan_mf.h:13; Ax_fan_mf.cpp:63). */
+void Ax_fan_mf_gpu_new_fb(float *X, float *y, float SO, float OD, float scale, int nx, int ny,
+                          int nv, float *sd_phi, int nd, float *y_det, int *id_X, int nt,
+                          float dy_det);
+
+/* 2D adjoint, finite-width beam (version 2).  Replaces Atx_fan_mf_gpu_new_fb
+ * (matlab/mex/src/Atx_fan_mf.h:13; Atx_fan_mf.cpp:67): cos_det / sin_det are
+ * the nd+1 edge tables the gateway reads from para.cos_det2 / sin_det2. */
+void Atx_fan_mf_gpu_new_fb(float *x, float *Y, float SO, float OD, float scale, int nx, int ny,
+                           float *sd_phi, int nd, float *y_det, float dy_det, int nt, int *id_Y,
+                           int *Nv, int tmp_size, int nv, float *cos_phi, float *sin_phi,
+                           float *cos_det, float *sin_det);
+
 /* 3D cone Siddon forward.  Replaces Ax_cone_mf_gpu_siddon
  * (matlab/mex/src/Ax_cone_mf.h:8; gateway matlab/mex/src/Ax_cone_mf.cpp:65).
  * Output slots follow the frame -> view lists id_Y/Nv, as the reference's
@@ -104,6 +118,9 @@ typedef struct fl_geometry {
     const int *id_X, *id_Y, *Nv;
     int tmp_size;
     const int *slot_view;
+    /* fan, finite-width beam (version 2) adjoint: cos/sin of the nd+1
+     * detector-cell edge angles (para.cos_det2 / sin_det2); optional */
+    const float *cos_det2, *sin_det2;
 } fl_geometry;
 
 typedef struct fl_plan_s *fl_plan;
@@ -122,9 +139,14 @@ int fl_plan_set_tile(fl_plan plan, int tile);
 /* adjoint kinds */
 #define FL_ADJ_EXACT 0   /* reference exact-chord voxel-driven adjoint (find_l / find_l_3d) */
 #define FL_ADJ_MATCHED 1 /* voxel-driven transpose of the Siddon forward (same lengths) */
+#define FL_ADJ_FAN_FB 2  /* fan, finite-width beam strip areas (Atx_fan_mf_cpu_new_fb.cpp:21-66);
+                            needs cos_det2/sin_det2; FP32 only */
 
 /* y = A x.  d_x: nt frames of nvox voxels, d_y: nv slots of ndet cells. */
 int fl_forward(fl_plan plan, const void *d_x, int x_dtype, void *d_y, int y_dtype, void *stream);
+/* Fan forward, finite-width beam: strip-area weights per detector cell
+ * (version 2, matlab/mex/src/Ax_fan_mf_cpu_new_fb.cpp:19-123); FP32. */
+int fl_forward_fan_fb(fl_plan plan, const float *d_x, float *d_y, void *stream);
 /* Fan forward with the Gao column walk (class default version 1,
  * matlab/mex/src/Ax_fan_mf_cpu_new.cpp:18-122); fan plans only. */
 int fl_forward_gao_fan(fl_plan plan, const void *d_x, int x_dtype, void *d_y, int y_dtype,

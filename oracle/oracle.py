@@ -81,6 +81,11 @@ def lib():
                                           _f, _f, _f, _f, _i, _i, _i, _i, _i, _i, _i]
         _lib.orc_chord_box_f64.restype = _d
         _lib.orc_chord_box_f64.argtypes = [_dp, _dp, _dp, _dp, _i]
+        _lib.orc_find_area.restype = _f
+        _lib.orc_find_area.argtypes = [_f] * 8
+        _lib.orc_fan_fb_fwd_f32.argtypes = [_fp, _fp, _f, _f, _f, _i, _i, _i, _fp, _i, _fp, _ip, _f]
+        _lib.orc_fan_fb_adj_f32.argtypes = [_fp, _fp, _f, _f, _f, _i, _i, _i, _f, _i, _ip, _ip, _i,
+                                            _fp, _fp, _fp, _fp]
     return _lib
 
 
@@ -104,6 +109,11 @@ def ref():
                                                  _f, _f, _f, _f, _i, _ip, _ip, _i, _i, _fp, _fp,
                                                  _i, _i]
         _ref.ref_Ax_fan_mf_cpu_new.argtypes = _ref.ref_Ax_fan_mf_cpu_siddon.argtypes
+        _ref.ref_find_area.restype = _f
+        _ref.ref_find_area.argtypes = [_f] * 8
+        _ref.ref_Ax_fan_mf_cpu_new_fb.argtypes = [_fp, _fp, _f, _f, _f, _i, _i, _i, _fp, _i, _fp, _ip, _f]
+        _ref.ref_Atx_fan_mf_cpu_new_fb.argtypes = [_fp, _fp, _f, _f, _f, _i, _i, _fp, _i, _fp, _f, _i,
+                                                   _ip, _ip, _i, _i, _fp, _fp, _fp, _fp]
     return _ref
 
 
@@ -262,6 +272,55 @@ def atx_fan_f32(Y, p):
                           p.y_os, p.nt, _pi(_ci(p.id_Y)), _pi(_ci(p.Nv)), p.tmp_size,
                           _pf(_c32(p.cos_phi)), _pf(_c32(p.sin_phi)), _pf(_c32(p.cos_det)),
                           _pf(_c32(p.sin_det)))
+    return x
+
+
+def find_area(a1, b1, c1, a2, b2, c2, x, y):
+    """Strip-area weight of a unit pixel (find_area.cpp:18-69)."""
+    return lib().orc_find_area(a1, b1, c1, a2, b2, c2, x, y)
+
+
+def fan_fb_fwd_f32(X, p):
+    """Finite-width-beam fan forward (Ax_fan_mf_cpu_new_fb.cpp:19-123)."""
+    X = _c32(X).ravel()
+    y = np.zeros(p.nv * p.nd, dtype=np.float32)
+    lib().orc_fan_fb_fwd_f32(_pf(X), _pf(y), p.SO, p.OD, p.scale, p.nx, p.ny, p.nv, _pf(_c32(p.sd_phi)),
+                             p.nd, _pf(_c32(p.y_det)), _pi(_ci(p.id_X)), p.dy_det)
+    return y
+
+
+def fan_fb_adj_f32(Y, p):
+    """Finite-width-beam fan adjoint (Atx_fan_mf_cpu_new_fb.cpp:21-66), edge tables
+    p.cos_det2 / p.sin_det2 (nd+1 entries)."""
+    Y = _c32(Y).ravel()
+    x = np.zeros(p.nx * p.ny * p.nt, dtype=np.float32)
+    lib().orc_fan_fb_adj_f32(_pf(x), _pf(Y), p.SO, p.OD, p.scale, p.nx, p.ny, p.nd, p.dy_det, p.nt,
+                             _pi(_ci(p.id_Y)), _pi(_ci(p.Nv)), p.tmp_size, _pf(_c32(p.cos_phi)),
+                             _pf(_c32(p.sin_phi)), _pf(_c32(p.cos_det2)), _pf(_c32(p.sin_det2)))
+    return x
+
+
+def ref_find_area(*args):
+    return ref().ref_find_area(*args)
+
+
+def ref_fan_fb_fwd(X, p):
+    X = _c32(X).ravel().copy()
+    y = np.zeros(p.nv * p.nd, dtype=np.float32)
+    ref().ref_Ax_fan_mf_cpu_new_fb(_pf(X), _pf(y), p.SO, p.OD, p.scale, p.nx, p.ny, p.nv,
+                                   _pf(_c32(p.sd_phi)), p.nd, _pf(_c32(p.y_det)), _pi(_ci(p.id_X)),
+                                   p.dy_det)
+    return y
+
+
+def ref_fan_fb_adj(Y, p):
+    Y = _c32(Y).ravel().copy()
+    x = np.zeros(p.nx * p.ny * p.nt, dtype=np.float32)
+    ref().ref_Atx_fan_mf_cpu_new_fb(_pf(x), _pf(Y), p.SO, p.OD, p.scale, p.nx, p.ny, _pf(_c32(p.sd_phi)),
+                                    p.nd, _pf(_c32(p.y_det)), p.dy_det, p.nt, _pi(_ci(p.id_Y)),
+                                    _pi(_ci(p.Nv)), p.tmp_size, p.nv, _pf(_c32(p.cos_phi)),
+                                    _pf(_c32(p.sin_phi)), _pf(_c32(p.cos_det2).copy()),
+                                    _pf(_c32(p.sin_det2).copy()))
     return x
 
 

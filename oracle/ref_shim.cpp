@@ -5,7 +5,8 @@
 // reference checkout (see oracle/Makefile).  The prototypes below restate the
 // signatures declared in the reference headers:
 //   matlab/mex/src/sort_alpha.h, Ax_fan_mf.h:2, Ax_cone_mf.h:7,
-//   find_l_3d.h, Atx_cone_mf.h:2, Ax_fan_mf.h:5 (Gao 2D forward).
+//   find_l_3d.h, Atx_cone_mf.h:2, Ax_fan_mf.h:5 (Gao 2D forward), find_area.h,
+//   Ax_fan_mf.h and Atx_fan_mf.h (finite-width beam, version 2).
 int sort_alpha(int Nx, int Ny, float *alpha_x, float *alpha_y, float *alpha);
 void Ax_fan_mf_cpu_siddon(float *X, float *y, float SO, float OD, float scale, int nx, int ny,
                           int nv, float *sd_phi, int nd, float *y_det, int *id_X);
@@ -21,6 +22,13 @@ void Atx_cone_mf_cpu_new(float *x, float *Y, float SO, float OD, float scale, in
                          int tmp_size, int nv, float *cos_phi, float *sin_phi, int na, int nb);
 void Ax_fan_mf_cpu_new(float *X, float *y, float SO, float OD, float scale, int nx, int ny,
                        int nv, float *sd_phi, int nd, float *y_det, int *id_X);
+float find_area(float a1, float b1, float c1, float a2, float b2, float c2, float x, float y);
+void Ax_fan_mf_cpu_new_fb(float *X, float *y, float SO, float OD, float scale, int nx, int ny,
+                          int nv, float *sd_phi, int nd, float *y_det, int *id_X, float dy_det);
+void Atx_fan_mf_cpu_new_fb(float *x, float *Y, float SO, float OD, float scale, int nx, int ny,
+                           float *sd_phi, int nd, float *y_det, float dy_det, int nt, int *id_Y,
+                           int *Nv, int tmp_size, int nv, float *cos_phi, float *sin_phi,
+                           float *cos_det, float *sin_det);
 
 extern "C" {
 int ref_sort_alpha(int Nx, int Ny, float *ax, float *ay, float *out)
@@ -61,5 +69,22 @@ void ref_Ax_fan_mf_cpu_new(float *X, float *y, float SO, float OD, float scale, 
                            int nv, float *sd_phi, int nd, float *y_det, int *id_X)
 {
     Ax_fan_mf_cpu_new(X, y, SO, OD, scale, nx, ny, nv, sd_phi, nd, y_det, id_X);
+}
+float ref_find_area(float a1, float b1, float c1, float a2, float b2, float c2, float x, float y)
+{
+    return find_area(a1, b1, c1, a2, b2, c2, x, y);
+}
+void ref_Ax_fan_mf_cpu_new_fb(float *X, float *y, float SO, float OD, float scale, int nx, int ny,
+                              int nv, float *sd_phi, int nd, float *y_det, int *id_X, float dy_det)
+{
+    Ax_fan_mf_cpu_new_fb(X, y, SO, OD, scale, nx, ny, nv, sd_phi, nd, y_det, id_X, dy_det);
+}
+void ref_Atx_fan_mf_cpu_new_fb(float *x, float *Y, float SO, float OD, float scale, int nx, int ny,
+                               float *sd_phi, int nd, float *y_det, float dy_det, int nt, int *id_Y,
+                               int *Nv, int tmp_size, int nv, float *cos_phi, float *sin_phi,
+                               float *cos_det, float *sin_det)
+{
+    Atx_fan_mf_cpu_new_fb(x, Y, SO, OD, scale, nx, ny, sd_phi, nd, y_det, dy_det, nt, id_Y, Nv,
+                          tmp_size, nv, cos_phi, sin_phi, cos_det, sin_det);
 }
 }

@@ -27,3 +27,42 @@ def first_mismatch(a, b):
         return None
     k = bad[0]
     return f"{bad.size} mismatches, first at {k}: {a[k]!r} vs {b[k]!r}"
+
+
+def clip_halfplane(poly, a, b, c, sign):
+    """Sutherland-Hodgman clip (FP64) of a convex polygon [(x, y), ...] to
+    sign*(a*x + b*y - c) >= 0."""
+    out = []
+    n = len(poly)
+    for i in range(n):
+        p, q = poly[i], poly[(i + 1) % n]
+        fp = sign * (a * p[0] + b * p[1] - c)
+        fq = sign * (a * q[0] + b * q[1] - c)
+        if fp >= 0:
+            out.append(p)
+        if (fp >= 0) != (fq >= 0):
+            t = fp / (fp - fq)
+            out.append((p[0] + t * (q[0] - p[0]), p[1] + t * (q[1] - p[1])))
+    return out
+
+
+def poly_area(poly):
+    if len(poly) < 3:
+        return 0.0
+    s = 0.0
+    for i in range(len(poly)):
+        x0, y0 = poly[i]
+        x1, y1 = poly[(i + 1) % len(poly)]
+        s += x0 * y1 - x1 * y0
+    return abs(s) / 2
+
+
+def strip_area(poly, l1, l2):
+    """Area (FP64) of `poly` between two lines a*x + b*y = c, i.e. where the
+    two line functions have opposite signs."""
+    tot = 0.0
+    for s in (1, -1):
+        q = clip_halfplane(poly, *l1, s)
+        q = clip_halfplane(q, *l2, -s)
+        tot += poly_area(q)
+    return tot
