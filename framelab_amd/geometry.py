@@ -29,6 +29,26 @@ def _frame_tables(nv: int):
     return id_X, id_Y, Nv, nv
 
 
+def with_frames(p, view_frame):
+    """Copy of a fan/cone para with several image frames (4D data): view iv
+    projects frame view_frame[iv].  The bookkeeping is setPara's for nt
+    frames (FanBeamScanner/setPara.m:32-52, ConeBeamScanner/setPara.m:59-80):
+    id_X per view, and per frame the list of its views in order, stored as
+    the columns of a tmp_size x nt matrix (id_Y) with counts Nv."""
+    import dataclasses
+    vf = np.asarray(view_frame, dtype=np.int32)
+    if vf.size != p.nv or vf.min() < 0:
+        raise ValueError("one non-negative frame index per view")
+    nt = int(vf.max()) + 1
+    lists = [np.nonzero(vf == f)[0].astype(np.int32) for f in range(nt)]
+    Nv = np.array([len(v) for v in lists], dtype=np.int32)
+    tmp = int(Nv.max())
+    id_Y = np.zeros((nt, tmp), dtype=np.int32)
+    for f, v in enumerate(lists):
+        id_Y[f, :v.size] = v
+    return dataclasses.replace(p, nt=nt, id_X=vf.copy(), id_Y=id_Y.ravel(), Nv=Nv, tmp_size=tmp)
+
+
 @dataclass
 class FanPara:
     """Fields of the fan-beam ``para`` struct (FanBeamScanner/setPara.m)."""
