@@ -17,7 +17,7 @@ FL_ERR_HIP = -2
 FL_ERR_ALLOC = -3
 FL_ERR_UNSUPPORTED = -4
 
-FL_F32, FL_BF16, FL_F16 = 0, 1, 2
+FL_F32, FL_BF16, FL_F16, FL_F64 = 0, 1, 2, 3
 FL_ADJ_EXACT, FL_ADJ_MATCHED, FL_ADJ_FAN_FB = 0, 1, 2
 
 _fp = C.POINTER(C.c_float)
@@ -38,6 +38,16 @@ class FlGeometry(C.Structure):
         ("tmp_size", C.c_int),
         ("slot_view", _ip),
         ("cos_det2", _fp), ("sin_det2", _fp),
+    ]
+
+
+class FlJohnGeometry(C.Structure):
+    """Mirror of ``fl_john_geometry`` (include/framelab_hip.h)."""
+    _fields_ = [
+        ("na", C.c_int), ("nb", C.c_int), ("nv", C.c_int), ("nz", C.c_int),
+        ("R", C.c_double), ("SO", C.c_double), ("h", C.c_double), ("dzeta", C.c_double),
+        ("dtheta", C.c_double), ("da", C.c_double), ("db", C.c_double),
+        ("a", C.POINTER(C.c_double)), ("b", C.POINTER(C.c_double)),
     ]
 
 
@@ -100,6 +110,13 @@ SIGNATURES = {
                                    C.c_int, _vp, _vp, C.POINTER(C.c_double), _vp]),
     "fl_plan_autotune": (C.c_int, [_vp, _ip, _fp]),
     "fl_plan_set_tile": (C.c_int, [_vp, C.c_int]),
+    "fl_forward_norm": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "fl_john_create": (C.c_int, [C.POINTER(_vp), C.POINTER(FlJohnGeometry)]),
+    "fl_john_destroy": (C.c_int, [_vp]),
+    "fl_john_sizes": (C.c_int, [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "fl_john_apply": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp]),
+    "fl_john_adjoint": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "fl_john_normal": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, _vp]),
     "fl_batch_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.POINTER(FlGeometry)]),
     "fl_batch_destroy": (C.c_int, [_vp]),
     "fl_batch_layout": (C.c_int, [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),

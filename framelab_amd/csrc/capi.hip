@@ -396,6 +396,15 @@ static int ensure_fb_tables(fl_plan_s* p)
     return FL_OK;
 }
 
+extern "C" int fl_forward_norm(fl_plan p, const float* d_y, float* d_yn, void* stream)
+{
+    if (!p || !d_y || !d_yn) return fl_fail(FL_ERR_INVALID, "null plan or buffer");
+    if (!p->cone) return fl_fail(FL_ERR_INVALID, "the normalised sinogram is a cone output");
+    hipError_t e = fl_launch_ynorm(d_y, d_yn, p->fg, (hipStream_t)stream);
+    if (e != hipSuccess) return fl_hip_fail(e, "yNorm launch");
+    return fl_ok();
+}
+
 extern "C" int fl_forward_fan_fb(fl_plan p, const float* d_x, float* d_y, void* stream)
 {
     if (!p || !d_x || !d_y) return fl_fail(FL_ERR_INVALID, "null plan or buffer");

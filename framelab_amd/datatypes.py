@@ -49,9 +49,6 @@ class ObjectData:
     def objdot(self, o) -> float:
         return float(torch.dot(self.dataArray.reshape(-1).double(), o.dataArray.reshape(-1).double()))
 
-    def norm(self) -> float:
-        return float(np.sqrt(self.objdot(self)))
-
 
 class CTData:
     """Sinogram data with the scanner that produced it (CTData.m:1-87)."""
@@ -65,3 +62,31 @@ class CTData:
 
     def objdot(self, o) -> float:
         return float(torch.dot(self.dataArray.reshape(-1).double(), o.dataArray.reshape(-1).double()))
+
+    def _john_input(self, smoothed):
+        """The normalised multi-helix sinogram (nHelix, nv, nb, na); with
+        `smoothed`, each helix replaced by the low-pass band of its one-level
+        3D linear framelet expansion (applyJohn.m:28-36)."""
+        from .framelets import FrameletSystem
+        g = self.dataArrayNorm
+        if g is None:
+            raise ValueError("CTData has no normalised data (dataArrayNorm)")
+        g = g.contiguous()
+        if smoothed:
+            fs = FrameletSystem(3, "linear", 1)
+            g = torch.stack([fs.forwardTransform(g[i].contiguous()).frameletArray[0, 0] for i in range(g.shape[0])])
+        return g.contiguous()
+
+    def applyJohn(self, smoothed: bool = False) -> torch.Tensor:
+        """Discretised John's equation D g of a multiHelix scan (applyJohn.m:1-106);
+        FP64, shape (nHelix-2 or 1, nv-2, nb-2, na-2)."""
+        from .john import JohnOperator, john_geometry
+        return JohnOperator(john_geometry(self.scanner)).apply(self._john_input(smoothed))
+
+    def applyJohnNormal(self, smoothed: bool = False) -> torch.Tensor:
+        """D^T D g (applyJohnNormal.m:1-156); FP64, shape (nHelix, nv, nb, na)."""
+        from .john import JohnOperator, john_geometry
+        return JohnOperator(john_geometry(self.scanner)).normal(self._john_input(smoothed))
+
+    def norm(self) -> float:
+        return float(np.sqrt(self.objdot(self)))

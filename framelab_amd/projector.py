@@ -156,6 +156,16 @@ class Projector:
                                          _stream()))
         return out
 
+    def forward_norm(self, y: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """yNorm = y / (|d| * scale) per ray (cone, FP32): the reference gateway's second output."""
+        self._check_tensor(y, self.ndet * self.nv, "y")
+        if out is None:
+            out = torch.empty_like(y)
+        self._check_tensor(out, self.ndet * self.nv, "out")
+        _lib.check(_lib.lib().fl_forward_norm(self._h, C.c_void_p(y.data_ptr()), C.c_void_p(out.data_ptr()),
+                                              _stream()))
+        return out
+
     def autotune(self):
         """Time the cone-forward ray-tile shapes on this geometry and keep the fastest.
         Returns (tile index, milliseconds of one forward with it)."""

@@ -97,10 +97,16 @@ class ConeBeamScanner(_Scanner):
         return self._projector(key, mk)
 
     def apply(self, obj: ObjectData) -> CTData:
+        """Forward scan and its normalised form yNorm = y/(|d| scale) (apply.m:1-58);
+        multiHelix data are split per helix, (nHelix, nv, nb, na)."""
         P = self.setPara(obj)
         x = obj.dataArray.to("cuda").float().contiguous()
         y = P.forward(x)
-        return CTData(self, y, None, obj.L)
+        yn = P.forward_norm(y)
+        if self.type == "multiHelix":
+            shape = (self.nHelix, self.para.nv // self.nHelix, self.nb, self.na)
+            y, yn = y.reshape(shape), yn.reshape(shape)
+        return CTData(self, y, yn, obj.L)
 
     def applyAdjoint(self, data: CTData, N=None) -> ObjectData:
         if self._pkey is None:

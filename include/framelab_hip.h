@@ -39,6 +39,7 @@ int fl_version(void);
 #define FL_F32 0
 #define FL_BF16 1
 #define FL_F16 2
+#define FL_F64 3 /* John's-equation input only */
 
 /* ------------------------------------------------ 1. drop-in entry points */
 
@@ -249,6 +250,38 @@ int fl_batch_split_bregman(fl_batch batch, int frame_kind, const float *d_f, flo
  * batch and keeps the fastest (outputs optional).  Must not overlap other
  * calls on the batch. */
 int fl_batch_autotune(fl_batch batch, int *fwd_tile, int *fwd_grid, int *adj_grid);
+
+/* ------------------------------------------------- John's equation
+ * Discretised John's equation on the normalised sinogram of a multi-helix
+ * scan (classes/+DataTypes/@CTData/applyJohn.m:38-106 and
+ * applyJohnNormal.m): g is (na, nb, nv views per helix, nz helices), a
+ * fastest; D g is (na-2, nb-2, nv-2, nz-2, or 1 with two helices), p
+ * fastest; FP64 results.  a, b: detector coordinates (host, na / nb
+ * doubles); R = source-detector distance; h = vtab/(2 pi rps); dzeta =
+ * phase shift between helices times h; dtheta = view step; da, db =
+ * detector pitches (physical units). */
+typedef struct {
+    int na, nb, nv, nz;
+    double R, SO, h, dzeta, dtheta, da, db;
+    const double *a, *b;
+} fl_john_geometry;
+typedef struct fl_john_s *fl_john;
+
+int fl_john_create(fl_john *john, const fl_john_geometry *g);
+int fl_john_destroy(fl_john john);
+int fl_john_sizes(fl_john john, long long *n_in, long long *n_out);
+/* out = D g (g: FL_F32 or FL_F64) */
+int fl_john_apply(fl_john john, const void *d_g, int g_dtype, double *d_out, void *stream);
+/* out = D^T u */
+int fl_john_adjoint(fl_john john, const double *d_u, double *d_out, void *stream);
+/* out = D^T D g; d_work holds n_out doubles */
+int fl_john_normal(fl_john john, const void *d_g, int g_dtype, double *d_work, double *d_out,
+                   void *stream);
+
+/* Normalised cone sinogram yNorm = y / (|d| * scale) per ray, the second
+ * output of the reference's cone gateway (matlab/mex/src/Ax_cone_mf.cpp:55-57);
+ * cone plans, FP32. */
+int fl_forward_norm(fl_plan plan, const float *d_y, float *d_ynorm, void *stream);
 
 /* Traversal log of selected rays (testing): rays = (view, ia, ib) triples
  * (host), cap entries per ray; J/S/cnt host outputs of nrays*cap / nrays. */

@@ -580,3 +580,71 @@ def split_bregman(A, At, W, Wt, lam, mu, f, u0, outer, cg_iters, cg_tol=0.0, shr
         v = v + (Wu - alpha)
         b = Atf + mu * Wt(alpha - v)
     return u
+
+
+# --------------------------------------------------------------------------
+# John's equation (classes/+DataTypes/@CTData/applyJohn.m:38-106): the
+# sparse operator assembled as the reference assembles it -- 1D difference
+# matrices (first differences (g[i+2]-g[i])/(2 step), second differences
+# (g[i]-2g[i+1]+g[i+2])/step^2, a forward difference across two helices),
+# Kronecker products in (helix, view, row, column) order, trims to the
+# interior, and the coefficient diagonals a, b, a^2 + R^2 -- in FP64.
+def john_matrix(na, nb, nv, nz, R, SO, h, dzeta, dtheta, da, db, a, b):
+    import scipy.sparse as sp
+
+    def eye(n):
+        return sp.identity(n, format="csr", dtype=np.float64)
+
+    def d1(n, step):
+        i = np.arange(n - 2)
+        return sp.csr_matrix((np.concatenate([-np.ones(n - 2), np.ones(n - 2)]),
+                              (np.concatenate([i, i]), np.concatenate([i, i + 2]))), shape=(n - 2, n)) / (2 * step)
+
+    def d2(n, step):
+        i = np.arange(n - 2)
+        return sp.csr_matrix((np.concatenate([np.ones(n - 2), -2 * np.ones(n - 2), np.ones(n - 2)]),
+                              (np.concatenate([i, i, i]), np.concatenate([i, i + 1, i + 2]))),
+                             shape=(n - 2, n)) / step ** 2
+
+    def K(z, v, bb, aa):
+        return sp.kron(sp.kron(z, v), sp.kron(bb, aa), format="csr")
+
+    def trim(n):
+        return eye(n)[1:-1, :]
+
+    Da1, Db1, Dbb1, Dt1 = d1(na, da), d1(nb, db), d2(nb, db), d1(nv, dtheta)
+    if nz > 2:
+        Dz1, zT, nzd = d1(nz, dzeta), trim(nz), nz - 2
+    else:
+        Dz1 = sp.csr_matrix(([-1.0, 1.0], ([0, 0], [0, 1])), shape=(1, nz)) / dzeta
+        zT, nzd = eye(nz)[0:1, :], nz - 1
+    Dtb = K(eye(nz), Dt1, Db1, eye(na))
+    Daz = K(Dz1, eye(nv), eye(nb), Da1)
+    Dbz = K(Dz1, eye(nv), Db1, eye(na))
+    Db = K(eye(nz), eye(nv), Db1, eye(na))
+    Dbb = K(eye(nz), eye(nv), Dbb1, eye(na))
+    Dab = K(eye(nz), eye(nv), Db1, Da1)
+    aT, bT, tT = trim(na), trim(nb), trim(nv)
+    azTrim = K(zT, eye(nv - 2), eye(nb - 2), aT)
+    btTrim = K(eye(nzd), tT, bT, eye(na - 2))
+    atTrim = K(eye(nzd), tT, eye(nb - 2), aT)
+    atzTrim = K(zT, tT, eye(nb - 2), aT)
+    tzTrim = K(zT, tT, eye(nb - 2), eye(na - 2))
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    A = np.broadcast_to(a[None, None, None, :], (nz, nv, nb - 2, na))
+    B = np.broadcast_to(b[1:-1][None, None, :, None], (nz, nv, nb - 2, na))
+    A2 = np.broadcast_to((a[1:-1] ** 2 + R ** 2)[None, None, None, :], (nz, nv, nb - 2, na - 2))
+    a1, b1, a2 = sp.diags(A.ravel()), sp.diags(B.ravel()), sp.diags(A2.ravel())
+    return (R * (azTrim @ Dtb) - R * SO * (btTrim @ Daz) - R * h * (atTrim @ Dbz)
+            + 2 * (atzTrim @ a1 @ Db) + atzTrim @ a1 @ b1 @ Dbb + tzTrim @ a2 @ Dab).tocsr()
+
+
+def john_apply(g, M):
+    """D g for g of shape (nz, nv, nb, na) (a fastest)."""
+    return M @ np.asarray(g, np.float64).ravel()
+
+
+def john_normal(g, M):
+    """D^T D g (applyJohnNormal.m:154)."""
+    return M.T @ (M @ np.asarray(g, np.float64).ravel())

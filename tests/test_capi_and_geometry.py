@@ -19,7 +19,7 @@ def _declared_functions():
             continue
         text = open(os.path.join(ROOT, "include", hdr)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        text = re.sub(r"typedef struct[^{]*\{.*?\}[^;]*;", "", text, flags=re.S)
+        text = re.sub(r"typedef struct[^{;]*\{.*?\}[^;]*;", "", text, flags=re.S)
         for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(", text, flags=re.M):
             if m.group(1) not in ("if", "while", "for", "defined"):
                 names.append(m.group(1))
