@@ -66,8 +66,7 @@ def cpu_baseline(p, X, views=3):
     kind = "reference" if O.has_ref() else "port"
     fwd = O.ref_siddon_cone if kind == "reference" else O.siddon_cone_f32
     adj = O.ref_atx_cone_cpu if kind == "reference" else O.atx_cone_f32
-    if kind == "port":
-        os.environ["OMP_NUM_THREADS"] = "1"
+    O.set_threads(1)
     t0 = time.perf_counter()
     y = fwd(X, sub)
     t1 = time.perf_counter()
@@ -76,9 +75,21 @@ def cpu_baseline(p, X, views=3):
     nvox = p.nx * p.ny * p.nz
     gups = 2.0 * nvox * views / ((t1 - t0) + (t2 - t1)) / 1e9
     model, cores = cpu_info()
-    return {"value": gups, "unit": "GUPS", "cores": 1, "kind": kind,
-            "sample": f"{views} of {p.nv} views at full 256^3 / 512x384 size, one forward + one adjoint "
-                      f"(t_A={t1 - t0:.2f}s, t_At={t2 - t1:.2f}s); host: {model}, {cores} logical CPUs"}
+    out = {"value": gups, "unit": "GUPS", "cores": 1, "kind": kind,
+           "sample": f"{views} of {p.nv} views at full 256^3 / 512x384 size, one forward + one adjoint "
+                     f"(t_A={t1 - t0:.2f}s, t_At={t2 - t1:.2f}s); host: {model}, {cores} logical CPUs"}
+    # the restatement (same algorithm, OpenMP over rays / voxels) on every core of this process
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    O.set_threads(ncpu)
+    t0 = time.perf_counter()
+    y = O.siddon_cone_f32(X, sub)
+    t1 = time.perf_counter()
+    O.atx_cone_f32(y, sub)
+    t2 = time.perf_counter()
+    O.set_threads(1)
+    out["all_cores"] = {"value": 2.0 * nvox * views / (t2 - t0) / 1e9, "unit": "GUPS", "cores": ncpu,
+                        "kind": "port", "sample": f"same sample, t_A={t1 - t0:.2f}s, t_At={t2 - t1:.2f}s"}
+    return out
 
 
 def measured_traffic(kernel):

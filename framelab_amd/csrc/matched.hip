@@ -64,7 +64,7 @@ __device__ __forceinline__ int mwave_max(int v)
 // z planes for each footprint row (each the forward's IEEE division, done
 // once per workgroup), the ray lengths |d| and the sinogram values; each
 // voxel then visits only the cells of its own shadow.  Views of each frame
-// in order, rows outer, columns inner; FP32 accumulation.
+// in order, columns outer; FP32 accumulation.
 // Six workgroups per CU (measured optimum of the register budget).
 template <typename YT, typename XT, int KZ>
 __global__ __launch_bounds__(256, 6) void k_adj_matched_cone(const YT* __restrict__ Y, XT* __restrict__ X,
@@ -168,7 +168,21 @@ __global__ __launch_bounds__(256, 6) void k_adj_matched_cone(const YT* __restric
                 t_y[k] = fl_load(y, (size_t)(B0 + kb) * na + (A0 + ka));
             }
             __syncthreads();
-            if (alo <= ahi) {
+            // columns outermost (no reference order to keep): the column's x/y
+            // slab interval is formed once and serves every row and voxel
+            for (int ia = alo; ia <= ahi; ia++) {
+                const int ka = ia - A0;
+                const float dx = t_dx[ka], dy = t_dy[ka];
+                SidAxis ax, ay;
+                sid_axis_init(ax, x1, dx, 1.0f, f.nx);
+                sid_axis_init(ay, y1, dy, 1.0f, f.ny);
+                const float x_lo = t_ax[ka][lx], x_hi = t_ax[ka][lx + 1];
+                const float y_lo = t_ay[ka][ly], y_hi = t_ay[ka][ly + 1];
+                const float inx = ax.on ? (dx > 0 ? x_lo : x_hi) : -FL_INF;
+                const float outx = ax.on ? (dx > 0 ? x_hi : x_lo) : FL_INF;
+                const float iny = ay.on ? (dy > 0 ? y_lo : y_hi) : -FL_INF;
+                const float outy = ay.on ? (dy > 0 ? y_hi : y_lo) : FL_INF;
+                if (!(fminf(outx, outy) > fmaxf(inx, iny))) continue;   // misses the (x, y) column
 #pragma unroll
                 for (int k = 0; k < KZ; k++) {
                     const int lz = lz0 + k, iz = iz0 + lz;
@@ -180,23 +194,10 @@ __global__ __launch_bounds__(256, 6) void k_adj_matched_cone(const YT* __restric
                         const float z_lo = t_az[kb][lz], z_hi = t_az[kb][lz + 1];
                         const float inz = az.on ? (dz > 0 ? z_lo : z_hi) : -FL_INF;
                         const float outz = az.on ? (dz > 0 ? z_hi : z_lo) : FL_INF;
-                        for (int ia = alo; ia <= ahi; ia++) {
-                            const int ka = ia - A0;
-                            const float dx = t_dx[ka], dy = t_dy[ka];
-                            SidAxis ax, ay;
-                            sid_axis_init(ax, x1, dx, 1.0f, f.nx);
-                            sid_axis_init(ay, y1, dy, 1.0f, f.ny);
-                            const float x_lo = t_ax[ka][lx], x_hi = t_ax[ka][lx + 1];
-                            const float y_lo = t_ay[ka][ly], y_hi = t_ay[ka][ly + 1];
-                            const float inx = ax.on ? (dx > 0 ? x_lo : x_hi) : -FL_INF;
-                            const float outx = ax.on ? (dx > 0 ? x_hi : x_lo) : FL_INF;
-                            const float iny = ay.on ? (dy > 0 ? y_lo : y_hi) : -FL_INF;
-                            const float outy = ay.on ? (dy > 0 ? y_hi : y_lo) : FL_INF;
-                            const int q = kb * nA + ka;
-                            const float Si = matched_coef(inx, outx, iny, outy, inz, outz, ax, ay, az, ix, iy, iz,
-                                                          dz_one, t_L[q]);
-                            if (Si != 0.0f) acc[k] = acc[k] + Si * t_y[q];
-                        }
+                        const int q = kb * nA + ka;
+                        const float Si = matched_coef(inx, outx, iny, outy, inz, outz, ax, ay, az, ix, iy, iz,
+                                                      dz_one, t_L[q]);
+                        if (Si != 0.0f) acc[k] = acc[k] + Si * t_y[q];
                     }
                 }
             }

@@ -786,3 +786,7 @@ void orc_fan_fb_adj_f32(float *x, const float *Y, float SO, float OD, float scal
         }
     }
 }
+
+/* Thread count of the OpenMP loops above (benchmarks time 1 and all cores). */
+#include <omp.h>
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }

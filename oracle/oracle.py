@@ -81,6 +81,7 @@ def lib():
                                           _f, _f, _f, _f, _i, _i, _i, _i, _i, _i, _i]
         _lib.orc_chord_box_f64.restype = _d
         _lib.orc_chord_box_f64.argtypes = [_dp, _dp, _dp, _dp, _i]
+        _lib.orc_set_threads.argtypes = [_i]
         _lib.orc_find_area.restype = _f
         _lib.orc_find_area.argtypes = [_f] * 8
         _lib.orc_fan_fb_fwd_f32.argtypes = [_fp, _fp, _f, _f, _f, _i, _i, _i, _fp, _i, _fp, _ip, _f]
@@ -648,3 +649,8 @@ def john_apply(g, M):
 def john_normal(g, M):
     """D^T D g (applyJohnNormal.m:154)."""
     return M.T @ (M @ np.asarray(g, np.float64).ravel())
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the restated kernels."""
+    lib().orc_set_threads(int(n))
