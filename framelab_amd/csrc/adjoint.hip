@@ -67,7 +67,8 @@ __global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y
     __shared__ float c_a[FP_A], c_b[FP_A], c_sba[FP_A], c_sab[FP_A];
     __shared__ float c_nx[FP_A], c_ny[FP_A], c_off[FP_A], c_hs[FP_A];
     __shared__ float c_ta[FP_A], c_tb[FP_A], c_t0[FP_A], c_hw[FP_A];
-    __shared__ float r_c[FP_B];
+    __shared__ float c_ra[FP_A], c_rb[FP_A];
+    __shared__ float r_c[FP_B], r_rc[FP_B];
     __shared__ float4 q_r[FP_A * FP_B];
     __shared__ float q_y[FP_A * FP_B];
     __shared__ int red[2][4][4];
@@ -163,8 +164,10 @@ __global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y
                 const float yd = g.ydet[A0 + k];
                 const float x2 = c * g.OD - s * yd, y2 = s * g.OD + c * yd;
                 const float a = x2 - x1, bb = y2 - y1;
+                const float ra = 1.0f / a, rb = 1.0f / bb;
                 c_a[k] = a; c_b[k] = bb;
-                c_sba[k] = bb / a; c_sab[k] = a / bb;
+                c_ra[k] = ra; c_rb[k] = rb;
+                c_sba[k] = fl_div_r(bb, a, ra); c_sab[k] = fl_div_r(a, bb, rb);
                 const float l2 = a * a + bb * bb, il = 1.0f / sqrtf(l2), il2 = 1.0f / l2;
                 c_nx[k] = bb * il; c_ny[k] = -a * il;
                 c_off[k] = bb * il * x1 - a * il * y1;
@@ -173,14 +176,18 @@ __global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y
                 c_t0[k] = -(x1 * a + y1 * bb) * il2;
                 c_hw[k] = (fl_absf(a) + fl_absf(bb)) * 0.5f * il2;
             }
-            for (int k = tid; k < nB; k += 256) r_c[k] = (g.zdet[B0 + k] + sz) - z1;
+            for (int k = tid; k < nB; k += 256) {
+                const float cc = (g.zdet[B0 + k] + sz) - z1;
+                r_c[k] = cc;
+                r_rc[k] = 1.0f / cc;
+            }
+            __syncthreads();
+            // the ratios' IEEE quotients from the staged reciprocals
             for (int k = tid; k < nA * nB; k += 256) {
                 const int ka = k % nA, kb = k / nA;
-                const float yd = g.ydet[A0 + ka];
-                const float x2 = c * g.OD - s * yd, y2 = s * g.OD + c * yd;
-                const float a = x2 - x1, bb = y2 - y1;
-                const float cc = (g.zdet[B0 + kb] + sz) - z1;
-                q_r[k] = make_float4(cc / a, cc / bb, a / cc, bb / cc);
+                const float a = c_a[ka], bb = c_b[ka], cc = r_c[kb], rc = r_rc[kb];
+                q_r[k] = make_float4(fl_div_r(cc, a, c_ra[ka]), fl_div_r(cc, bb, c_rb[ka]), fl_div_r(a, cc, rc),
+                                     fl_div_r(bb, cc, rc));
                 q_y[k] = fl_load(y, (size_t)(B0 + kb) * na + (A0 + ka));
             }
             __syncthreads();

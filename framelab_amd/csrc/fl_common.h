@@ -23,6 +23,22 @@
 struct bf16_t { uint16_t v; };
 struct f16_t { _Float16 v; };
 
+// num / d given r = 1.0f / d (correctly rounded), bit-identical to the IEEE
+// quotient: q = num*r lies within an ulp of it, the fused residual
+// num - q*d is exact, and q + residual*r rounds to the quotient
+// (Markstein's theorem, for operands whose quotient and residual neither
+// overflow nor underflow, as geometry differences do not).  d = +-0 makes r
+// infinite; the product num*r is then IEEE's signed infinity (or NaN for
+// 0/0), which the fused step would turn into NaN, so it is kept.  Lets one
+// division per denominator serve a whole table of quotients.
+__device__ __forceinline__ float fl_div_r(float num, float d, float r)
+{
+    const float q = num * r;
+    const float e = __builtin_fmaf(-q, d, num);
+    const float v = __builtin_fmaf(e, r, q);
+    return __builtin_isinf(r) ? q : v;
+}
+
 __device__ __forceinline__ float fl_load(const float* p, size_t i) { return p[i]; }
 __device__ __forceinline__ float fl_load(const bf16_t* p, size_t i)
 {

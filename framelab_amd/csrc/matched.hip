@@ -72,8 +72,8 @@ __global__ __launch_bounds__(256, 6) void k_adj_matched_cone(const YT* __restric
 {
     constexpr int BZ = 4 * KZ;
     __shared__ float t_ax[MF_A][9], t_ay[MF_A][9];
-    __shared__ float t_dx[MF_A], t_dy[MF_A];
-    __shared__ float t_az[MF_B][BZ + 1], t_dz[MF_B];
+    __shared__ float t_dx[MF_A], t_dy[MF_A], t_rx[MF_A], t_ry[MF_A];
+    __shared__ float t_az[MF_B][BZ + 1], t_dz[MF_B], t_rz[MF_B];
     __shared__ float t_L[MF_A * MF_B], t_y[MF_A * MF_B];
     __shared__ int red[2][4][4];
 
@@ -144,26 +144,32 @@ __global__ __launch_bounds__(256, 6) void k_adj_matched_cone(const YT* __restric
         const float x1 = c * (-f.SO), y1 = s * (-f.SO), z1 = sz;
         if (nA <= MF_A && nB <= MF_B) {
             // columns: direction components and the brick's x/y plane parameters
+            for (int k = tid; k < nA; k += 256) {
+                const float yd = f.ydet[A0 + k];
+                const float dx = (c * f.OD - s * yd) - x1, dy = (s * f.OD + c * yd) - y1;
+                t_dx[k] = dx; t_dy[k] = dy;
+                t_rx[k] = 1.0f / dx; t_ry[k] = 1.0f / dy;
+            }
+            for (int k = tid; k < nB; k += 256) {
+                const float dz = (f.zdet[B0 + k] + sz) - z1;
+                t_dz[k] = dz;
+                t_rz[k] = 1.0f / dz;
+            }
+            __syncthreads();
+            // plane coordinate i -> ((float)i - o)/d, the forward's IEEE
+            // quotient, from the staged reciprocal of d
             for (int k = tid; k < nA * 9; k += 256) {
                 const int ka = k / 9, q = k % 9;
-                const float yd = f.ydet[A0 + ka];
-                const float dx = (c * f.OD - s * yd) - x1, dy = (s * f.OD + c * yd) - y1;
-                // plane coordinate i -> ((float)i - o)/d, as the forward computes it
-                t_ax[ka][q] = ((float)(ix0 + q - f.nx / 2) - x1) / dx;
-                t_ay[ka][q] = ((float)(iy0 + q - f.ny / 2) - y1) / dy;
-                if (q == 0) { t_dx[ka] = dx; t_dy[ka] = dy; }
+                t_ax[ka][q] = fl_div_r((float)(ix0 + q - f.nx / 2) - x1, t_dx[ka], t_rx[ka]);
+                t_ay[ka][q] = fl_div_r((float)(iy0 + q - f.ny / 2) - y1, t_dy[ka], t_ry[ka]);
             }
             for (int k = tid; k < nB * (BZ + 1); k += 256) {
                 const int kb = k / (BZ + 1), q = k % (BZ + 1);
-                const float dz = (f.zdet[B0 + kb] + sz) - z1;
-                t_az[kb][q] = ((float)(iz0 + q - f.nz / 2) * f.dz0 - z1) / dz;
-                if (q == 0) t_dz[kb] = dz;
+                t_az[kb][q] = fl_div_r((float)(iz0 + q - f.nz / 2) * f.dz0 - z1, t_dz[kb], t_rz[kb]);
             }
             for (int k = tid; k < nA * nB; k += 256) {
                 const int ka = k % nA, kb = k / nA;
-                const float yd = f.ydet[A0 + ka];
-                const float dx = (c * f.OD - s * yd) - x1, dy = (s * f.OD + c * yd) - y1;
-                const float dz = (f.zdet[B0 + kb] + sz) - z1;
+                const float dx = t_dx[ka], dy = t_dy[ka], dz = t_dz[kb];
                 t_L[k] = sqrtf(dx * dx + dy * dy + dz * dz);
                 t_y[k] = fl_load(y, (size_t)(B0 + kb) * na + (A0 + ka));
             }
