@@ -42,10 +42,8 @@ def ellipses_2d(n: int, count: int = 10, seed: int = 0) -> np.ndarray:
     return img.astype(np.float32)
 
 
-def ellipsoids_3d(n, count: int = 20, seed: int = 0) -> np.ndarray:
-    """float32 volume [nz, ny, nx] of `count` additive random ellipsoids.
-    n is an int (cube) or a (nx, ny, nz) triple."""
-    nx, ny, nz = (n, n, n) if np.isscalar(n) else n
+def _ellipsoid_params(count: int, seed: int):
+    """(centre, semi-axes, rotation, density) of `count` seeded ellipsoids."""
     rng = np.random.default_rng(seed)
     params = []
     for _ in range(count):
@@ -54,6 +52,14 @@ def ellipsoids_3d(n, count: int = 20, seed: int = 0) -> np.ndarray:
         R = _haar_rotation(rng)
         rho = rng.uniform(0.1, 1.0)
         params.append((ctr, ax, R, rho))
+    return params
+
+
+def ellipsoids_3d(n, count: int = 20, seed: int = 0) -> np.ndarray:
+    """float32 volume [nz, ny, nx] of `count` additive random ellipsoids.
+    n is an int (cube) or a (nx, ny, nz) triple."""
+    nx, ny, nz = (n, n, n) if np.isscalar(n) else n
+    params = _ellipsoid_params(count, seed)
     cx = -1.0 + (np.arange(nx) + 0.5) * 2.0 / nx
     cy = -1.0 + (np.arange(ny) + 0.5) * 2.0 / ny
     cz = -1.0 + (np.arange(nz) + 0.5) * 2.0 / nz
@@ -68,4 +74,29 @@ def ellipsoids_3d(n, count: int = 20, seed: int = 0) -> np.ndarray:
             q = (P - ctr) @ R          # coordinates in the ellipsoid frame
             sl[((q / ax) ** 2).sum(-1) <= 1.0] += rho
         vol[iz] = sl
+    return vol
+
+
+def ellipsoids_3d_device(n: int, count: int = 20, seed: int = 0, device="cuda"):
+    """The ellipsoids of ``ellipsoids_3d(n, count, seed)`` rasterised on a
+    torch device, z-slab by z-slab, for volumes too large for the host loop
+    (512^3 and up).  FP64 coordinates as on the host; voxels exactly on an
+    ellipsoid's surface may round to the other side of it, so the volume is
+    a phantom of the same family, not the host's bit for bit."""
+    import torch
+    params = _ellipsoid_params(count, seed)
+    c = -1.0 + (torch.arange(n, dtype=torch.float64, device=device) + 0.5) * 2.0 / n
+    vol = torch.zeros((n, n, n), dtype=torch.float32, device=device)
+    slab = max(1, (1 << 24) // (n * n))
+    Yg, Xg = torch.meshgrid(c, c, indexing="ij")
+    for z0 in range(0, n, slab):
+        zs = c[z0:z0 + slab]
+        P = torch.stack([Xg.expand(zs.numel(), n, n), Yg.expand(zs.numel(), n, n),
+                         zs[:, None, None].expand(zs.numel(), n, n)], -1)
+        acc = torch.zeros((zs.numel(), n, n), dtype=torch.float64, device=device)
+        for ctr, ax, R, rho in params:
+            q = (P - torch.from_numpy(ctr).to(device)) @ torch.from_numpy(R).to(device)
+            inside = ((q / torch.from_numpy(ax).to(device)) ** 2).sum(-1) <= 1.0
+            acc += inside.to(torch.float64) * rho
+        vol[z0:z0 + slab] = acc.to(torch.float32)
     return vol
