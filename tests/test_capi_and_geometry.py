@@ -37,6 +37,30 @@ def test_library_exports_every_declared_symbol():
     assert set(declared) <= set(_lib.SIGNATURES), set(declared) - set(_lib.SIGNATURES)
 
 
+REF_MEX = "/root/reference/matlab/mex/src"
+REF_HEADERS = ("Ax_fan_mf.h", "Atx_fan_mf.h", "Atx_fan_mf_cpu.h", "Ax_cone_mf.h", "Atx_cone_mf.h")
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_MEX), reason="reference checkout absent")
+def test_dropins_match_the_gateway_headers(tmp_path):
+    """The mex gateways link against the drop-ins unchanged only if every
+    GPU function their headers declare extern "C" is exported here with the
+    same parameter list.  One translation unit includes this library's
+    header and the gateways' headers: C-linkage functions cannot be
+    overloaded, so any differing declaration is a compile error."""
+    import subprocess
+    gpu_funcs = set()
+    for h in REF_HEADERS:
+        gpu_funcs |= set(re.findall(r'extern\s+"C"\s+void\s+(\w+)\s*\(', open(os.path.join(REF_MEX, h)).read()))
+    assert len(gpu_funcs) == 8, gpu_funcs
+    assert gpu_funcs <= set(_declared_functions()), gpu_funcs - set(_declared_functions())
+    tu = tmp_path / "gateway_headers.cpp"
+    tu.write_text('#include "framelab_hip.h"\n' + "".join(f'#include "{h}"\n' for h in REF_HEADERS))
+    r = subprocess.run(["g++", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), "-I", REF_MEX, str(tu)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
 def test_error_strings_without_gpu():
     from framelab_amd import _lib
     L = _lib.lib()
