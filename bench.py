@@ -261,7 +261,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 2D leg")
     ap.add_argument("--profile", action="store_true",
                     help="counter run: one forward, one exact and one matched adjoint at 256^3, then a "
-                         "plain 1 GiB device copy (the calibration kernel for HBM byte counters); no JSON")
+                         "plain 1 GiB device copy (the calibration kernel for HBM byte counters) and the "
+                         "FMA-chain kernel (the calibration for VALU instruction counters); no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); multi-GPU sharding is "
@@ -304,8 +305,15 @@ def main():
         src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
         dst.copy_(src)
+        # VALU calibration: 8192 workgroups of FP32 FMA chains, then of integer chains
+        from framelab_amd import _lib
+        import ctypes as C
+        out = torch.empty(8192 * 256, dtype=torch.float32, device=dev)
+        for kind in (0, 1):
+            _lib.check(_lib.lib().fl_calibrate_valu(C.c_void_p(out.data_ptr()), out.numel(), 5000, kind,
+                                                    C.c_void_p(torch.cuda.current_stream().cuda_stream)))
         torch.cuda.synchronize()
-        log("bench: profile run done (forward, exact adjoint, matched adjoint, 1 GiB copy)")
+        log("bench: profile run done (forward, exact adjoint, matched adjoint, 1 GiB copy, VALU calibration)")
         return
 
     log("bench: warmup")

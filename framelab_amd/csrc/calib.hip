@@ -1,0 +1,55 @@
+// calib.hip -- VALU calibration kernel for counter-based rooflines.
+//
+// The projectors are bound by vector-ALU issue, not HBM.  To state what
+// fraction of the VALU peak they reach, the profiler's per-dispatch VALU
+// instruction count is compared with these kernels' under the same
+// counters: eight independent chains per thread of FP32 FMAs or of 32-bit
+// integer shift/add/xor, no memory traffic but one store per thread, so
+// each issues VALU instructions of its kind at the rate the hardware allows.
+#include <hip/hip_runtime.h>
+
+#include "../../include/framelab_hip.h"
+#include "fl_capi_internal.h"
+
+namespace {
+
+// KIND 0: FP32 FMA chains; KIND 1: 32-bit integer shift/add/xor chains.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_valu_peak(float* __restrict__ out, int iters, float b, float c)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    float a[8];
+    unsigned u[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        a[j] = (float)threadIdx.x + (float)j;
+        u[j] = threadIdx.x + (unsigned)j;
+    }
+    for (int k = 0; k < iters; k++) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            if (KIND == 0) a[j] = __builtin_fmaf(a[j], b, c);
+            else u[j] = (u[j] >> 1) ^ (u[j] + (unsigned)iters);
+        }
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 8; j++) s += KIND == 0 ? a[j] : (float)u[j];
+    out[i] = s;
+}
+
+}  // namespace
+
+extern "C" int fl_calibrate_valu(float* d_out, int n_threads, int iters, int kind, void* stream)
+{
+    if (!d_out || n_threads <= 0 || (n_threads % 256) != 0 || iters <= 0 || (kind != 0 && kind != 1))
+        return fl_fail(FL_ERR_INVALID, "n_threads must be a positive multiple of 256, iters positive, kind 0 or 1");
+    const dim3 grid(n_threads / 256);
+    if (kind == 0)
+        hipLaunchKernelGGL(k_valu_peak<0>, grid, dim3(256), 0, (hipStream_t)stream, d_out, iters, 0.999f, 1e-3f);
+    else
+        hipLaunchKernelGGL(k_valu_peak<1>, grid, dim3(256), 0, (hipStream_t)stream, d_out, iters, 0.999f, 1e-3f);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fl_hip_fail(e, "fl_calibrate_valu");
+    return fl_ok();
+}

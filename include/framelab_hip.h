@@ -287,6 +287,14 @@ int fl_forward_norm(fl_plan plan, const float *d_y, float *d_ynorm, void *stream
  * (host), cap entries per ray; J/S/cnt host outputs of nrays*cap / nrays. */
 int fl_siddon_trace(fl_plan plan, int nrays, const int *rays, int cap, int *J, float *S, int *cnt);
 
+/* Measurement helper (no reference counterpart): eight independent chains
+ * of `iters` steps per thread, of FP32 FMAs (kind 0) or of 32-bit integer
+ * shift/add/xor (kind 1), one store per thread to d_out (n_threads floats, a
+ * positive multiple of 256).  Their per-dispatch VALU instruction rates
+ * under the profiler are the issue peaks the projectors' rates are compared
+ * with. */
+int fl_calibrate_valu(float *d_out, int n_threads, int iters, int kind, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
