@@ -335,3 +335,44 @@ def test_helical_adjoints(gpu, oracle, kind):
     lhs = float(np.dot(y.astype(np.float64), Y.ravel()))
     rhs = float(np.dot(X.ravel().astype(np.float64), xm))
     assert abs(lhs - rhs) <= 1e-5 * abs(lhs), (lhs, rhs)
+
+
+def _trace_transpose(oracle, p, Y):
+    """A^T y from the oracle's FP32 traversal logs of every ray, summed in
+    FP64: the coefficients the matched adjoint must reproduce."""
+    rays = np.stack(np.meshgrid(np.arange(p.nv), np.arange(p.na), np.arange(p.nb), indexing="ij"), -1)
+    rays = rays.reshape(-1, 3)
+    out = np.zeros(p.nz * p.ny * p.nx, np.float64)
+    for k, (J, S) in enumerate(oracle.siddon_trace(p, rays, cone=True)):
+        iv, ia, ib = rays[k]
+        np.add.at(out, J, S.astype(np.float64) * float(Y[iv, ib, ia]))
+    return out * float(p.scale)
+
+
+@pytest.mark.parametrize("geom", ["staged", "all_wide", "mixed"])
+def test_adjoints_staged_and_direct_paths(gpu, oracle, geom):
+    """Detector pitches fine against the voxels, so the per-workgroup
+    footprint of a view exceeds the staged tables for every brick
+    ("all_wide": 16^3 against 256 x 192 cells) or for the bricks near the
+    source only ("mixed": 32^3 against 160 x 100), exercising the direct path
+    of both adjoints beside the table path ("staged": the usual pitch, 32^3
+    against 64 x 48, tables only).  Exact adjoint bit-exact against
+    the oracle; matched adjoint against the FP64 sum of the oracle's FP32
+    traversal coefficients (the transpose it equals coefficient by
+    coefficient; only the FP32 accumulation differs)."""
+    from framelab_amd.projector import ADJ_MATCHED
+    _say(f"case: adjoint footprint paths ({geom})")
+    p = {"staged": lambda: small_cone(nv=8), "all_wide": lambda: small_cone(256, 192, 4, 16),
+         "mixed": lambda: small_cone(160, 100, 6, 32)}[geom]()
+    rng = np.random.default_rng(21)
+    Y = rng.random((p.nv, p.nb, p.na), dtype=np.float32)
+    P = _proj(p)
+    yt = torch.from_numpy(Y).to(gpu)
+    x = P.adjoint(yt).cpu().numpy().ravel()
+    xo = oracle.atx_cone_f32(Y, p)
+    assert bits_equal(x, xo), first_mismatch(x, xo)
+    xm = P.adjoint(yt, kind=ADJ_MATCHED).double().cpu().numpy().ravel()
+    want = _trace_transpose(oracle, p, Y)
+    err = np.abs(xm - want).max() / np.abs(want).max()
+    _say(f"  matched vs traced transpose: max rel {err:.2e}")
+    assert err <= 1e-5
