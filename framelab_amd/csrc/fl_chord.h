@@ -4,10 +4,14 @@
 // adjoints are bit-identical to the reference's:
 //   fl_chord_line2  <- matlab/mex/src/find_l.cpp:18-45 (line a*x+b*y=c through
 //                      the unit pixel at (x,y));
-//   fl_chord_seg2   <- matlab/mex/src/find_l_3d.cpp:64-137 (segment-line
+//   fl_chord_seg2   <- matlab/mex/src/find_l_3d.cpp:19-92 (segment-line
 //                      through a dx-by-dy cell);
-//   fl_chord_box3   <- matlab/mex/src/find_l_3d.cpp:139-227 (line through a
+//   fl_chord_box3   <- matlab/mex/src/find_l_3d.cpp:94-182 (line through a
 //                      dx-by-dy-by-dz box; faces tried x,y,z or y,x,z).
+// One deliberate difference: a line parallel to the x = const (y = const)
+// planes is credited only to boxes whose x (y) slab holds it; the
+// reference's fallback to the 2-D routine (find_l_3d.cpp:104-108) ignores
+// that coordinate, crediting boxes the line misses.
 #pragma once
 
 #include "fl_common.h"
@@ -103,8 +107,10 @@ FL_HD float fl_chord_box3(float x1, float y1, float z1, float x2, float y2, floa
                           float dy, float dz, float x, float y, float z)
 {
     float a = x2 - x1, b = y2 - y1, c = z2 - z1;
-    if (a == 0) return fl_chord_seg2(y1, z1, y2, z2, dy, dz, y, z);
-    if (b == 0) return fl_chord_seg2(x1, z1, x2, z2, dx, dz, x, z);
+    // the line lies in the plane x = x1 (y = y1): outside the box's slab
+    // along that axis it misses the box
+    if (a == 0) return fl_absf(x1 - x) <= dx / 2 ? fl_chord_seg2(y1, z1, y2, z2, dy, dz, y, z) : 0.0f;
+    if (b == 0) return fl_absf(y1 - y) <= dy / 2 ? fl_chord_seg2(x1, z1, x2, z2, dx, dz, x, z) : 0.0f;
     const float p[3] = {x1 - x, y1 - y, z1 - z};
     const float d[3] = {a, b, c};
     const float h[3] = {dx / 2, dy / 2, dz / 2};

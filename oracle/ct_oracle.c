@@ -15,7 +15,7 @@
  *   matlab/mex/src/Ax_fan_mf_cpu_siddon.cpp:19-93  2D Siddon forward
  *   matlab/mex/src/Ax_cone_mf_cpu_siddon.cpp:19-134 3D Siddon forward
  *   matlab/mex/src/find_l.cpp:18-45                2D line/pixel chord
- *   matlab/mex/src/find_l_3d.cpp:64-227            3D ray/voxel chord
+ *   matlab/mex/src/find_l_3d.cpp:19-182            3D ray/voxel chord
  *   matlab/mex/src/Atx_fan_mf_cpu_new.cpp:21-63    2D pixel-driven adjoint
  *   matlab/mex/src/Atx_cone_mf_gpu_new.cu:41-84    3D voxel-driven adjoint
  *
@@ -398,7 +398,7 @@ float orc_find_l(float a, float b, float c, float x, float y)
 }
 
 /* Chord of the segment-line (x1,y1)->(x2,y2) through the dx-by-dy cell
- * centred at (x,y).  find_l_3d.cpp:64-137 (2D helper). */
+ * centred at (x,y).  find_l_3d.cpp:19-92 (2D helper). */
 float orc_find_l2(float x1_0, float y1_0, float x2_0, float y2_0, float dx, float dy,
                   float x, float y)
 {
@@ -416,7 +416,7 @@ float orc_find_l2(float x1_0, float y1_0, float x2_0, float y2_0, float dx, floa
     }
     float px = x1_0 - x, py = y1_0 - y;
     /* x-faces: y at x=-hx and x=+hx; y-faces: x at y=-hy and y=+hy.  The
-     * face order depends on which of |a|,|b| dominates (find_l_3d.cpp:87-128). */
+     * face order depends on which of |a|,|b| dominates (find_l_3d.cpp:42-83). */
     int xfirst = FABS(a) > FABS(b);
     for (int pass = 0; pass < 2 && k < 2; pass++) {
         int xf = (pass == 0) ? xfirst : !xfirst;
@@ -441,7 +441,7 @@ float orc_find_l2(float x1_0, float y1_0, float x2_0, float y2_0, float dx, floa
  * `ax` (0=x,1=y,2=z) at offsets -h[ax], +h[ax] from the box centre.  p = line
  * point relative to the centre, d = direction, h = half sides.  `second`
  * marks the 2nd/3rd pass, where the + face is only tried while fewer than
- * two points were found (find_l_3d.cpp:167-187). */
+ * two points were found (find_l_3d.cpp:114-173). */
 static void face_pair(int ax, const float *p, const float *d, const float *h, int second,
                       float pts[2][3], int *k)
 {
@@ -461,15 +461,18 @@ static void face_pair(int ax, const float *p, const float *d, const float *h, in
 }
 
 /* Chord of the line (x1,y1,z1)->(x2,y2,z2) through the box centred at
- * (x,y,z) with sides (dx,dy,dz).  find_l_3d.cpp:139-227: a==0 or b==0 fall
- * back to the 2D routine; otherwise faces are tried x,y,z (|a|>|b|) or
- * y,x,z, stopping once two points are found. */
+ * (x,y,z) with sides (dx,dy,dz).  find_l_3d.cpp:94-182: a==0 or b==0 fall
+ * back to the 2D routine (:104-108); otherwise faces are tried x,y,z
+ * (|a|>|b|) or y,x,z, stopping once two points are found (:114-173).
+ * Deviation: with a==0 (b==0) the line lies in the plane x = x1 (y = y1)
+ * and is credited only to boxes whose x (y) slab holds it; the reference's
+ * 2D fallback ignores that coordinate and credits boxes the line misses. */
 float orc_find_l_3d(float x1_0, float y1_0, float z1_0, float x2_0, float y2_0, float z2_0,
                     float dx, float dy, float dz, float x, float y, float z)
 {
     float a = x2_0 - x1_0, b = y2_0 - y1_0, c = z2_0 - z1_0;
-    if (a == 0) return orc_find_l2(y1_0, z1_0, y2_0, z2_0, dy, dz, y, z);
-    if (b == 0) return orc_find_l2(x1_0, z1_0, x2_0, z2_0, dx, dz, x, z);
+    if (a == 0) return FABS(x1_0 - x) <= dx / 2 ? orc_find_l2(y1_0, z1_0, y2_0, z2_0, dy, dz, y, z) : 0.0f;
+    if (b == 0) return FABS(y1_0 - y) <= dy / 2 ? orc_find_l2(x1_0, z1_0, x2_0, z2_0, dx, dz, x, z) : 0.0f;
     float p[3] = {x1_0 - x, y1_0 - y, z1_0 - z};
     float d[3] = {a, b, c};
     float h[3] = {dx / 2, dy / 2, dz / 2};

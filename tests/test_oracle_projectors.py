@@ -44,10 +44,25 @@ def test_golden_cone(oracle):
             assert bits_equal(x, GOLD[f"cone_adj_{k}"]), first_mismatch(x, GOLD[f"cone_adj_{k}"])
 
 
+def _misses_parallel_slab(a):
+    """A line parallel to the x = const (y = const) planes whose plane lies
+    outside the box's x (y) slab: the chord is 0; the reference's 2-D
+    fallback ignores that coordinate (find_l_3d.cpp:104-108)."""
+    x1, y1, _, x2, y2, _, dx, dy, _, x, y, _ = map(np.float32, a)
+    if x2 - x1 == 0:
+        return abs(x1 - x) > dx / 2
+    if y2 - y1 == 0:
+        return abs(y1 - y) > dy / 2
+    return False
+
+
 def test_golden_find_l_3d(oracle):
     args = cases.find_l_3d_args()
     got = np.array([oracle.find_l_3d(*map(float, a)) for a in args], np.float32)
-    assert bits_equal(got, GOLD["find_l_3d"]), first_mismatch(got, GOLD["find_l_3d"])
+    miss = np.array([_misses_parallel_slab(a) for a in args])
+    assert miss.sum() > 5 and np.all(got[miss] == 0.0)
+    assert np.any(GOLD["find_l_3d"][miss] != 0.0)   # the reference credits some of them
+    assert bits_equal(got[~miss], GOLD["find_l_3d"][~miss]), first_mismatch(got[~miss], GOLD["find_l_3d"][~miss])
     # known answers of matlab/mex/src2/test_find_l_3d.cpp: a z-directed ray
     # misses the box at (5,5,5) and runs its full dz=0.25 through (0,0,0.5)
     assert got[0] == 0.0 and got[1] == np.float32(0.25)
@@ -118,6 +133,24 @@ def _box_chord_np(p0, p1, c, h):
         a, b = (c[k] - h[k] - p0[k]) / d[k], (c[k] + h[k] - p0[k]) / d[k]
         t0, t1 = max(t0, min(a, b)), min(t1, max(a, b))
     return max(0.0, t1 - t0) * float(np.linalg.norm(d))
+
+
+def test_chord_axis_parallel_lines_closed_form(oracle):
+    """Lines in an x = const or y = const plane (the 2-D fallback of the 3-D
+    chord), inside and outside the box's slab along that axis, against FP64
+    slab clipping."""
+    rng = np.random.default_rng(14)
+    for k in range(600):
+        s = rng.uniform(-6, 6, 3).astype(np.float32).astype(np.float64)
+        e = rng.uniform(-6, 6, 3).astype(np.float32).astype(np.float64)
+        c = rng.uniform(-1, 1, 3).astype(np.float32).astype(np.float64)
+        ax = k % 2
+        e[ax] = s[ax]
+        if k % 3 == 0:
+            s[ax] = e[ax] = np.float32(c[ax] + rng.uniform(-0.5, 0.5))   # inside the slab
+        want = _box_chord_np(s, e, c, np.array([0.5, 0.5, 0.5]))
+        got = oracle.find_l_3d(*s, *e, 1.0, 1.0, 1.0, *c)
+        assert abs(got - want) <= 2e-5 * (1 + abs(want)), (s, e, c, got, want)
 
 
 def test_chord_functions_closed_form(oracle):
