@@ -9,7 +9,7 @@
 #include "fl_siddon.h"
 
 // Siddon ray sum of fan ray (view iv, cell id), scaled
-// (Ax_fan_mf_cpu_siddon.cpp:92-151).
+// (Ax_fan_mf_cpu_siddon.cpp:32-91).
 template <typename VT>
 __device__ __forceinline__ float fl_fan_ray_sum(const VT* __restrict__ x, const FwdGeom& g, int iv, int id)
 {

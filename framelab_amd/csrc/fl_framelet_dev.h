@@ -65,7 +65,7 @@ __device__ __forceinline__ void fl_bands2(const FlTaps& T, const float (&nb)[NT]
 }
 
 // Synthesis at one pixel: sum_i corr_x[R_i] sum_j corr_y[R_j] C_ij
-// (FraRecMultiLevel2D.m:203-211); band b of C starts at C + b*band_stride.
+// (FraRecMultiLevel2D.m:41-49); band b of C starts at C + b*band_stride.
 template <int NF, int NT, typename CT>
 __device__ __forceinline__ float fl_synth2(const CT* __restrict__ C, size_t band_stride, int nx, int ny, int x, int y,
                                            int step, const FlTaps& T)

@@ -2,7 +2,7 @@
 // traversal semantics (device code).
 //
 // Reference semantics (matlab/mex/src/Ax_cone_mf_cpu_siddon.cpp:19-134,
-// Ax_fan_mf_cpu_siddon.cpp:79-153, sort_alpha.cpp:15-60):
+// Ax_fan_mf_cpu_siddon.cpp:19-93, sort_alpha.cpp:15-60):
 //   * plane parameters alpha = (i*s - o)/d for planes i = -n/2..n/2 of each
 //     axis with d != 0 (s = 1 for x and y, s = dz0 for z), in float;
 //   * the sorted union of all axes' parameters (an exact cross-axis tie is

@@ -7,9 +7,9 @@
 //   1D pass  utils/ConvSymAsym3D.m:19-24: circular correlation with the taps
 //            dilated by 2^(L-1), kernel centre floor((len+1)/2);
 //   2D       FraDecMultiLevel2D.m:43-51: band(i,j) = corr_y[D_j] corr_x[D_i] u;
-//            FraRecMultiLevel2D.m:203-211: u = sum_i corr_x[R_i] sum_j corr_y[R_j] C_ij;
-//   3D       FraDecMultiLevel3D.m:88-99: band(i,j,k) = corr_z[D_k] corr_x[D_j] corr_y[D_i] u;
-//            FraRecMultiLevel3D.m:246-258: u = sum_i corr_y[R_i] sum_j corr_x[R_j]
+//            FraRecMultiLevel2D.m:41-49: u = sum_i corr_x[R_i] sum_j corr_y[R_j] C_ij;
+//   3D       FraDecMultiLevel3D.m:35-47: band(i,j,k) = corr_z[D_k] corr_x[D_j] corr_y[D_i] u;
+//            FraRecMultiLevel3D.m:35-48: u = sum_i corr_y[R_i] sum_j corr_x[R_j]
 //            sum_k corr_z[R_k] C_ijk.
 // Band b = i + NF*j (+ NF*NF*k) (MATLAB linear order of the cell {i,j,k});
 // coefficients are stored band-major: C[b][z][y][x].

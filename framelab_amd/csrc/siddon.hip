@@ -6,7 +6,7 @@
 // consecutive patches of a view run on the same XCD and share its L2.
 //
 // Arithmetic follows matlab/mex/src/Ax_cone_mf_cpu_siddon.cpp:35-131 and
-// Ax_fan_mf_cpu_siddon.cpp:92-151 operation by operation (this file is built
+// Ax_fan_mf_cpu_siddon.cpp:32-91 operation by operation (this file is built
 // with -ffp-contract=off), so ray sums are bit-identical to the reference CPU
 // path; see fl_siddon.h for the traversal.
 #include <algorithm>

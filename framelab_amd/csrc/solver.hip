@@ -532,7 +532,7 @@ extern "C" int fl_framelet_adjoint(int dim, int kind, int level, int nx, int ny,
         if (e != hipSuccess) return fl_hip_fail(e, "framelet adjoint");
         return fl_ok();
     }
-    // FraRecMultiLevel2D.m:177-181 / 3D.m:225-228: C{L-1}{1} <- Rec(C{L}),
+    // FraRecMultiLevel2D.m:16-18 / 3D.m:14-16: C{L-1}{1} <- Rec(C{L}),
     // ..., u = Rec(C{1}).  The caller's coefficients are left untouched:
     // each rebuilt level lives in one half of the work buffer (2*nb*n FP32),
     // its low-pass band replaced by the reconstruction of the level above.

@@ -22,7 +22,8 @@ f32 = np.float32
 
 
 def _frame_tables(nv: int):
-    """Single-frame bookkeeping (setPara.m:59-80 cone, :32-52 fan)."""
+    """Single-frame bookkeeping (ConeBeamScanner/setPara.m:59-80,
+    FanBeamScanner/setPara.m:32-52)."""
     id_X = np.zeros(nv, dtype=np.int32)
     id_Y = np.arange(nv, dtype=np.int32)
     Nv = np.array([nv], dtype=np.int32)
@@ -131,7 +132,7 @@ def _cos_sin_single(phi32: np.ndarray):
 
 
 def _det_coords(n: int, pitch: np.float32, offset: np.float32, scale: np.float32):
-    """single(((-n/2:n/2-1)+0.5)*pitch+offset)/scale (setPara.m:53-54)."""
+    """single(((-n/2:n/2-1)+0.5)*pitch+offset)/scale (FanBeamScanner/setPara.m:23)."""
     k = (np.arange(-(n // 2), n - n // 2, dtype=np.float64) + 0.5).astype(f32)
     return ((k * f32(pitch)).astype(f32) + f32(offset)).astype(f32) / f32(scale)
 
@@ -174,7 +175,7 @@ def cone_para(na: int, nb: int, nv: int, N=(256, 256, 256), L=(2.0, 2.0, 2.0),
               SO=5.0, OD=5.0, Ly=6.0, Lz=6.0, y_os=0.0, kind: str = "circle",
               zmax=2.0, rps=3.0, vtab=2.0, fps=30.0, nHelix=1, phaseShift=(0.0,),
               sd_phi: np.ndarray | None = None) -> ConePara:
-    """ConeBeamScanner(kind,na,nb,nv,...).setPara(object) (setPara.m:1-83).
+    """ConeBeamScanner(kind,na,nb,nv,...).setPara(object) (ConeBeamScanner/setPara.m:1-83).
 
     ``kind`` is 'circle', 'helix' or 'multiHelix' (ConeBeamScanner.m:38-40);
     helical trajectories derive nv from fps*zmax/vtab as the constructor does

@@ -97,7 +97,7 @@ class ConeBeamScanner(_Scanner):
         return self._projector(key, mk)
 
     def apply(self, obj: ObjectData) -> CTData:
-        """Forward scan and its normalised form yNorm = y/(|d| scale) (apply.m:1-58);
+        """Forward scan and its normalised form yNorm = y/(|d| scale) (ConeBeamScanner/apply.m:1-57);
         multiHelix data are split per helix, (nHelix, nv, nb, na)."""
         P = self.setPara(obj)
         x = obj.dataArray.to("cuda").float().contiguous()

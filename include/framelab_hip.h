@@ -44,18 +44,18 @@ int fl_version(void);
 /* ------------------------------------------------ 1. drop-in entry points */
 
 /* 2D fan Siddon forward.  Replaces Ax_fan_mf_gpu_siddon
- * (matlab/mex/src/Ax_fan_mf.h:3; gateway call matlab/mex/src/Ax_fan_mf.cpp:49).
+ * (matlab/mex/src/Ax_fan_mf.h:3; gateway call matlab/mex/src/Ax_fan_mf.cpp:51).
  * Output y[iv*nd+id]; image frame of view iv is id_X[iv]. */
 void Ax_fan_mf_gpu_siddon(float *X, float *y, float SO, float OD, float scale, int nx, int ny,
                           int nv, float *sd_phi, int nd, float *y_det, int *id_X, int nt);
 
 /* 2D fan forward, class default version 1 (Gao column walk).  Replaces
- * Ax_fan_mf_gpu_new (matlab/mex/src/Ax_fan_mf.h:6; Ax_fan_mf.cpp:55). */
+ * Ax_fan_mf_gpu_new (matlab/mex/src/Ax_fan_mf.h:7; Ax_fan_mf.cpp:57). */
 void Ax_fan_mf_gpu_new(float *X, float *y, float SO, float OD, float scale, int nx, int ny,
                        int nv, float *sd_phi, int nd, float *y_det, int *id_X, int nt);
 
 /* 2D pixel-driven exact adjoint.  Replaces Atx_fan_mf_gpu_new
- * (matlab/mex/src/Atx_fan_mf.h:3; gateway matlab/mex/src/Atx_fan_mf.cpp:60). */
+ * (matlab/mex/src/Atx_fan_mf.h:4; gateway matlab/mex/src/Atx_fan_mf.cpp:59). */
 void Atx_fan_mf_gpu_new(float *x, float *Y, float SO, float OD, float scale, int nx, int ny,
                         float *sd_phi, int nd, float *y_det, float dy_det, float y_os, int nt,
                         int *id_Y, int *Nv, int tmp_size, int nv, float *cos_phi, float *sin_phi,
@@ -76,7 +76,7 @@ void Atx_fan_mf_gpu_new_fb(float *x, float *Y, float SO, float OD, float scale, 
                            float *cos_det, float *sin_det);
 
 /* 3D cone Siddon forward.  Replaces Ax_cone_mf_gpu_siddon
- * (matlab/mex/src/Ax_cone_mf.h:8; gateway matlab/mex/src/Ax_cone_mf.cpp:65).
+ * (matlab/mex/src/Ax_cone_mf.h:10; gateway matlab/mex/src/Ax_cone_mf.cpp:65).
  * Output slots follow the frame -> view lists id_Y/Nv, as the reference's
  * GPU wrapper orders them (matlab/mex/src/Ax_cone_mf_gpu_siddon.cu:167-186). */
 void Ax_cone_mf_gpu_siddon(float *X, float *y, float *sd_phi, float *sd_z, float *y_det,
@@ -86,7 +86,7 @@ void Ax_cone_mf_gpu_siddon(float *X, float *y, float *sd_phi, float *sd_z, float
 
 /* 3D cone forward, class default version 1, with the normalised second
  * output yNorm = y / (|source - detector cell| * scale).  Replaces
- * Ax_cone_mf_gpu_new (matlab/mex/src/Ax_cone_mf.h:5; Ax_cone_mf.cpp:71).
+ * Ax_cone_mf_gpu_new (matlab/mex/src/Ax_cone_mf.h:3; Ax_cone_mf.cpp:71).
  * y is the exact Siddon ray sum (the reference's Gao cone column walk
  * misplaces lengths when the y and z slopes have opposite signs). */
 void Ax_cone_mf_gpu_new(float *X, float *y, float *yNorm, float *sd_phi, float *sd_z,
@@ -95,7 +95,7 @@ void Ax_cone_mf_gpu_new(float *X, float *y, float *yNorm, float *sd_phi, float *
                         int nv, int tmp_size, int nv_block);
 
 /* 3D voxel-driven exact adjoint.  Replaces Atx_cone_mf_gpu_new
- * (matlab/mex/src/Atx_cone_mf.h:3; gateway matlab/mex/src/Atx_cone_mf.cpp:64). */
+ * (matlab/mex/src/Atx_cone_mf.h:4; gateway matlab/mex/src/Atx_cone_mf.cpp:65). */
 void Atx_cone_mf_gpu_new(float *X, float *y, float *cos_phi, float *sin_phi, float *sd_z,
                          float *y_det, float *z_det, int *id_Y, int *Nv, float SO, float OD,
                          float scale, float dy_det, float y_os, float dz_det, float dz, int nx,

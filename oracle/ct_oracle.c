@@ -89,7 +89,7 @@ static int ray_alphas(int nx, int ny, int nz, float dz0,
 
 /* Walk the merged list: each segment's voxel is the floor of its midpoint,
  * the length is (a[i+1]-a[i])*|d| (Ax_cone_mf_cpu_siddon.cpp:114-128).
- * nz==0 selects the 2D walk (Ax_fan_mf_cpu_siddon.cpp:137-148).
+ * nz==0 selects the 2D walk (Ax_fan_mf_cpu_siddon.cpp:77-88).
  * When J/S are non-null the in-volume segments are logged in order. */
 static float walk(int n_alpha, const float *al, int nx, int ny, int nz, float dz0,
                   float x1, float y1, float z1, float dx, float dy, float dz,
@@ -124,13 +124,13 @@ static float walk(int n_alpha, const float *al, int nx, int ny, int nz, float dz
 }
 
 /* Per-view trig the way the reference's C++ computes it: (float)cos(float)
- * resolves to the float overload (Ax_fan_mf_cpu_siddon.cpp:94). */
+ * resolves to the float overload (Ax_fan_mf_cpu_siddon.cpp:34). */
 void orc_trig_f32(int nv, const float *phi, float *c, float *s)
 {
     for (int i = 0; i < nv; i++) { c[i] = cosf(phi[i]); s[i] = sinf(phi[i]); }
 }
 
-/* 2D fan Siddon, Ax_fan_mf_cpu_siddon.cpp:79-153.  cos_phi/sin_phi are the
+/* 2D fan Siddon, Ax_fan_mf_cpu_siddon.cpp:19-93.  cos_phi/sin_phi are the
  * per-view values from orc_trig_f32. */
 void orc_siddon_fan_f32(const float *X, float *y, float SO, float OD, float scale,
                         int nx, int ny, int nv, const float *cos_phi, const float *sin_phi,
@@ -654,7 +654,7 @@ float orc_find_area(float a1, float b1, float c1, float a2, float b2, float c2, 
 /* Column range of a beam bounded by the source rays through two edge
  * points, over the unit column [u1, u1+1] of the major axis: the four edge
  * crossings floor(o + slope_k*(u - o_major) + n/2), kept in [0, n-1];
- * (lo, hi) start at (n-1, 0) (Ax_fan_mf_cpu_new_fb.cpp:62-73). */
+ * (lo, hi) start at (n-1, 0) (Ax_fan_mf_cpu_new_fb.cpp:59-75). */
 static void beam_range(float o, float om, const float *slope, float u1, int n, int *lo, int *hi)
 {
     int n2 = n / 2;

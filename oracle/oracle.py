@@ -156,7 +156,7 @@ def trig_f32(phi):
 
 
 def siddon_fan_f32(X, p):
-    """2D fan Siddon (Ax_fan_mf_cpu_siddon.cpp:79-153); pinned bit-exact to _ref."""
+    """2D fan Siddon (Ax_fan_mf_cpu_siddon.cpp:19-93); pinned bit-exact to _ref."""
     X = _c32(X).ravel()
     c, s = trig_f32(p.sd_phi)
     y = np.zeros(p.nv * p.nd, dtype=np.float32)
@@ -647,7 +647,7 @@ def john_apply(g, M):
 
 
 def john_normal(g, M):
-    """D^T D g (applyJohnNormal.m:154)."""
+    """D^T D g (applyJohnNormal.m:155)."""
     return M.T @ (M @ np.asarray(g, np.float64).ravel())
 
 

@@ -4,8 +4,8 @@
 // together with the reference sources, compiled where they lie under the
 // reference checkout (see oracle/Makefile).  The prototypes below restate the
 // signatures declared in the reference headers:
-//   matlab/mex/src/sort_alpha.h, Ax_fan_mf.h:2, Ax_cone_mf.h:7,
-//   find_l_3d.h, Atx_cone_mf.h:2, Ax_fan_mf.h:5 (Gao 2D forward), find_area.h,
+//   matlab/mex/src/sort_alpha.h, Ax_fan_mf.h:2, Ax_cone_mf.h:9,
+//   find_l_3d.h, Atx_cone_mf.h:2, Ax_fan_mf.h:6 (Gao 2D forward), find_area.h,
 //   Ax_fan_mf.h and Atx_fan_mf.h (finite-width beam, version 2).
 int sort_alpha(int Nx, int Ny, float *alpha_x, float *alpha_y, float *alpha);
 void Ax_fan_mf_cpu_siddon(float *X, float *y, float SO, float OD, float scale, int nx, int ny,
