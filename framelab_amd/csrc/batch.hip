@@ -419,6 +419,12 @@ extern "C" int fl_batch_create(fl_batch* out, int nprob, const fl_geometry* geom
             return rc;
         }
         B->plans.push_back(pl);
+        if (pl->det_dev > FL_DET_DEV_MAX) {
+            b_free(B);
+            delete B;
+            return fl_fail(FL_ERR_UNSUPPORTED, "problem %d: detector coordinates stray %.3g cells from a uniform "
+                                               "grid (the batched adjoint needs uniform spacing)", i, pl->det_dev);
+        }
         BDesc d = {};
         d.f = pl->fg;
         d.a = pl->ag;

@@ -239,6 +239,13 @@ extern "C" int fl_plan_create(fl_plan* out, const fl_geometry* g)
         f.inv_pa = a.inv_pa = (float)(1.0 / pa);
         f.zb0 = a.zb0 = cone ? g->z_det[0] : 0.0f;
         f.inv_pb = a.inv_pb = (float)(1.0 / pb);
+        // deviation from the uniform grids the window maps assume
+        double dev = 0.0;
+        for (int k = 0; k < g->na && pa != 0.0; k++)
+            dev = fmax(dev, fabs(((double)g->y_det[k] - g->y_det[0]) / pa - k));
+        for (int k = 0; cone && k < g->nb && pb != 0.0; k++)
+            dev = fmax(dev, fabs(((double)g->z_det[k] - g->z_det[0]) / pb - k));
+        p->det_dev = dev;
         const double rvol = 0.5 * sqrt((double)g->nx * g->nx + (double)g->ny * g->ny) + 1.0;
         if (!(pa != 0.0 && pb != 0.0) || !(g->SO > rvol)) {
             plan_free_arrays(p);
@@ -420,6 +427,10 @@ extern "C" int fl_adjoint(fl_plan p, const void* d_y, int yt, void* d_x, int xt,
 {
     if (!p || !d_x || !d_y) return fl_fail(FL_ERR_INVALID, "null plan or buffer");
     if (!dtype_ok(xt) || !dtype_ok(yt)) return fl_fail(FL_ERR_INVALID, "bad dtype");
+    if (p->det_dev > FL_DET_DEV_MAX)
+        return fl_fail(FL_ERR_UNSUPPORTED,
+                       "detector coordinates stray %.3g cells from a uniform grid; the voxel-driven adjoints "
+                       "need uniform spacing (the forward does not)", p->det_dev);
     hipStream_t st = (hipStream_t)stream;
     hipError_t e;
     if (kind == FL_ADJ_EXACT) {

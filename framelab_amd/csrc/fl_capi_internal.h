@@ -25,12 +25,19 @@ struct fl_plan_s {
     float *d_fb_px = nullptr, *d_fb_py = nullptr, *d_fb_pd = nullptr;
     float *d_cdet2 = nullptr, *d_sdet2 = nullptr;
     std::vector<float> h_ydet;
+    // largest distance, in cells, of a detector coordinate from the uniform
+    // grid through the first and last cell: the adjoints' window maps assume
+    // a uniform pitch
+    double det_dev = 0.0;
     // lazily allocated workspace (solvers, gradients)
     void* ws = nullptr;
     size_t ws_bytes = 0;
 };
 
 constexpr int FL_FWD_NTILES = 6;   // cone forward ray-tile shapes (siddon.hip)
+// detector coordinates may stray from a uniform grid by at most this many
+// cells for the voxel-driven adjoints (their windows widen by 0.05 cells)
+constexpr double FL_DET_DEV_MAX = 0.01;
 
 int fl_fail(int code, const char* fmt, ...);
 int fl_ok();

@@ -202,6 +202,27 @@ def test_invalid_arguments_raise(gpu):
         P.forward(torch.zeros(10, device=gpu))
 
 
+def test_nonuniform_detector(gpu, oracle):
+    """A detector column moved by 0.3 of a pitch: the ray-driven forward
+    takes any coordinates (bit-exact against the oracle), the voxel-driven
+    adjoints, whose windows assume a uniform pitch, refuse with an error
+    instead of silently dropping cells."""
+    from framelab_amd import _lib
+    from framelab_amd.projector import ADJ_MATCHED
+    _say("case: non-uniform detector spacing")
+    p = small_cone(nv=6)
+    p.y_det = p.y_det.copy()
+    p.y_det[10] += np.float32(0.3) * (p.y_det[11] - p.y_det[10])
+    X = np.random.default_rng(31).random((p.nz, p.ny, p.nx), dtype=np.float32)
+    P = _proj(p)
+    y = P.forward(torch.from_numpy(X).to(gpu)).cpu().numpy().ravel()
+    assert bits_equal(y, oracle.siddon_cone_f32(X, p))
+    yt = torch.ones(P.sino_shape, device=gpu)
+    for kind in (None, ADJ_MATCHED):
+        with pytest.raises(_lib.FrameLabError, match="uniform"):
+            P.adjoint(yt) if kind is None else P.adjoint(yt, kind=kind)
+
+
 def test_lowp_storage(gpu):
     _say("case: bf16/fp16 storage forward + adjoint")
     p = small_cone()
