@@ -75,6 +75,34 @@ static void plan_free_arrays(fl_plan_s* p)
                     p->d_fb_px, p->d_fb_py, p->d_fb_pd, p->d_cdet2, p->d_sdet2};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
+    if (p->fwd_done) (void)hipEventDestroy(p->fwd_done);
+}
+
+// Cone forwards of one plan share its brick buffer, so each is ordered
+// after the previous one, whatever streams they run on: wait for the event
+// the previous forward recorded, record it again after this one.  A stream
+// under graph capture is left alone (external events cannot enter a
+// capture); the solvers that capture forwards order their private stream
+// against the caller's.
+static int fwd_order_begin(fl_plan_s* p, hipStream_t st, bool& order)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess) cs = hipStreamCaptureStatusNone;
+    order = p->fwd_done != nullptr && cs == hipStreamCaptureStatusNone;
+    if (order && p->fwd_pending) {
+        const hipError_t e = hipStreamWaitEvent(st, p->fwd_done, 0);
+        if (e != hipSuccess) return fl_hip_fail(e, "forward ordering");
+    }
+    return FL_OK;
+}
+
+static int fwd_order_end(fl_plan_s* p, hipStream_t st, bool order)
+{
+    if (!order) return FL_OK;
+    const hipError_t e = hipEventRecord(p->fwd_done, st);
+    if (e != hipSuccess) return fl_hip_fail(e, "forward ordering");
+    p->fwd_pending = true;
+    return FL_OK;
 }
 
 extern "C" int fl_plan_create(fl_plan* out, const fl_geometry* g)
@@ -207,6 +235,8 @@ extern "C" int fl_plan_create(fl_plan* out, const fl_geometry* g)
         // padded to whole bricks; 4-byte elements cover every storage type)
         const size_t fr = (size_t)((g->nx + 7) / 8) * ((g->ny + 7) / 8) * 64 * (size_t)g->nz;
         if (hipMalloc(&p->d_bricks, fr * (size_t)g->nt * 4) != hipSuccess) rc = fl_fail(FL_ERR_ALLOC, "brick buffer");
+        else if (hipEventCreateWithFlags(&p->fwd_done, hipEventDisableTiming) != hipSuccess)
+            rc = fl_fail(FL_ERR_HIP, "forward ordering event");
     }
     if (rc) {
         plan_free_arrays(p);
@@ -305,12 +335,17 @@ extern "C" int fl_plan_autotune(fl_plan p, int* fwd_tile, float* fwd_ms)
     float *x = nullptr, *y = nullptr;
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    // the timed forwards use the plan's brick buffer: after any forward in
+    // flight, and none may start meanwhile
+    std::lock_guard<std::mutex> lk(p->fwd_mu);
     int rc = FL_OK;
     if (hipMalloc((void**)&x, nxb) != hipSuccess || hipMalloc((void**)&y, nyb) != hipSuccess)
         rc = fl_fail(FL_ERR_ALLOC, "autotune buffers (%zu + %zu bytes)", nxb, nyb);
     if (!rc && (hipStreamCreate(&st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
                 hipEventCreate(&e1) != hipSuccess))
         rc = fl_fail(FL_ERR_HIP, "autotune stream/events");
+    if (!rc && p->fwd_pending && hipStreamWaitEvent(st, p->fwd_done, 0) != hipSuccess)
+        rc = fl_fail(FL_ERR_HIP, "autotune ordering");
     int best = p->fg.tile;
     float best_ms = 1e30f;
     if (!rc) {
@@ -348,6 +383,7 @@ extern "C" int fl_plan_autotune(fl_plan p, int* fwd_tile, float* fwd_ms)
 extern "C" int fl_plan_set_tile(fl_plan p, int tile)
 {
     if (!p || tile < 0 || tile >= FL_FWD_NTILES) return fl_fail(FL_ERR_INVALID, "bad plan or tile %d", tile);
+    std::lock_guard<std::mutex> lk(p->fwd_mu);
     p->fg.tile = tile;
     return fl_ok();
 }
@@ -357,10 +393,20 @@ extern "C" int fl_forward(fl_plan p, const void* d_x, int xt, void* d_y, int yt,
     if (!p || !d_x || !d_y) return fl_fail(FL_ERR_INVALID, "null plan or buffer");
     if (!dtype_ok(xt) || !dtype_ok(yt)) return fl_fail(FL_ERR_INVALID, "bad dtype");
     hipStream_t st = (hipStream_t)stream;
-    hipError_t e = p->cone ? fl_launch_fwd_cone(d_x, xt, d_y, yt, p->fg, st)
-                           : fl_launch_fwd_fan(d_x, xt, d_y, yt, p->fg, st);
+    if (!p->cone) {
+        const hipError_t e = fl_launch_fwd_fan(d_x, xt, d_y, yt, p->fg, st);
+        if (e == hipErrorInvalidValue) return fl_fail(FL_ERR_UNSUPPORTED, "dtype combination %d/%d", xt, yt);
+        if (e != hipSuccess) return fl_hip_fail(e, "forward launch");
+        return fl_ok();
+    }
+    std::lock_guard<std::mutex> lk(p->fwd_mu);
+    bool order = false;
+    int rc = fwd_order_begin(p, st, order);
+    if (rc) return rc;
+    const hipError_t e = fl_launch_fwd_cone(d_x, xt, d_y, yt, p->fg, st);
     if (e == hipErrorInvalidValue) return fl_fail(FL_ERR_UNSUPPORTED, "dtype combination %d/%d", xt, yt);
     if (e != hipSuccess) return fl_hip_fail(e, "forward launch");
+    if ((rc = fwd_order_end(p, st, order))) return rc;
     return fl_ok();
 }
 
@@ -380,7 +426,8 @@ extern "C" int fl_forward_gao_fan(fl_plan p, const void* d_x, int xt, void* d_y,
 // Py = cosf(angle + phi), Pd = SO*sinf(angle).
 static int ensure_fb_tables(fl_plan_s* p)
 {
-    if (p->d_fb_px) return FL_OK;
+    std::lock_guard<std::mutex> lk(p->fwd_mu);
+    if (p->d_fb_px && p->d_fb_py && p->d_fb_pd) return FL_OK;
     const int nv = p->nv, nd = (int)p->ndet;
     const float SO = p->fg.SO, SD = p->fg.SO + p->fg.OD, dy = p->ag.dy_det;
     std::vector<float> px((size_t)nv * nd * 2), py((size_t)nv * nd * 2), pd((size_t)nd * 2);
@@ -398,8 +445,14 @@ static int ensure_fb_tables(fl_plan_s* p)
     int rc;
     if ((rc = upload(&p->d_fb_px, px.data(), px.size(), "strip Px")) ||
         (rc = upload(&p->d_fb_py, py.data(), py.size(), "strip Py")) ||
-        (rc = upload(&p->d_fb_pd, pd.data(), pd.size(), "strip Pd")))
+        (rc = upload(&p->d_fb_pd, pd.data(), pd.size(), "strip Pd"))) {
+        // leave no partial set behind: the next call builds all three again
+        for (float** q : {&p->d_fb_px, &p->d_fb_py, &p->d_fb_pd}) {
+            if (*q) (void)hipFree(*q);
+            *q = nullptr;
+        }
         return rc;
+    }
     return FL_OK;
 }
 

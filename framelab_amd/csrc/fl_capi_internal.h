@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <vector>
 
 #include "fl_geom.h"
@@ -20,6 +21,11 @@ struct fl_plan_s {
     float *d_cphi = nullptr, *d_sphi = nullptr, *d_cdet = nullptr, *d_sdet = nullptr;
     int *d_slot_view = nullptr, *d_view_frame = nullptr, *d_idY = nullptr, *d_Nv = nullptr;
     void* d_bricks = nullptr;   // cone: bricked copy of the forward input (nt frames, 8x8 bricks, 4-byte)
+    // forwards share d_bricks: each one waits for the previous one (on
+    // whatever stream it ran) through this event, recorded after it
+    hipEvent_t fwd_done = nullptr;
+    bool fwd_pending = false;
+    std::mutex fwd_mu;
     // fan, finite-width beam: strip lines per (view, cell, edge) and cell
     // edge tables (built on first use / uploaded when given)
     float *d_fb_px = nullptr, *d_fb_py = nullptr, *d_fb_pd = nullptr;

@@ -444,3 +444,25 @@ def test_axis_aligned_rays(gpu, oracle):
     xm = P.adjoint(yt, kind=ADJ_MATCHED).double().cpu().numpy().ravel()
     want = _trace_transpose(oracle, p, Y)
     assert np.abs(xm - want).max() <= 1e-5 * np.abs(want).max()
+
+
+def test_forwards_on_two_streams(gpu):
+    """Cone forwards of one plan share its bricked copy of the input; calls
+    alternating between two streams without host synchronisation are
+    ordered by the plan, so each sinogram equals the one computed alone."""
+    _say("case: one plan, forwards alternating between two streams")
+    p = small_cone(nv=24)
+    P = _proj(p)
+    g = torch.Generator(device="cpu").manual_seed(41)
+    xs = [torch.rand(P.vol_shape, generator=g).to(gpu) for _ in range(6)]
+    want = [P.forward(x).cpu() for x in xs]
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    outs = [torch.empty(P.sino_shape, device=gpu) for _ in xs]
+    for rep in range(3):
+        for k, x in enumerate(xs):
+            with torch.cuda.stream(streams[k % 2]):
+                P.forward(x, out=outs[k])
+        torch.cuda.synchronize()
+        for k in range(len(xs)):
+            assert torch.equal(outs[k].cpu(), want[k]), (rep, k)
