@@ -26,19 +26,18 @@ __global__ __launch_bounds__(256) void k_adj_fan(const YT* __restrict__ Y, XT* _
 
 // Cone, footprint-staged: per view the workgroup stages its detector
 // footprint in LDS -- the rays' direction components, the six direction
-// ratios find_l_3d divides out (each computed once per workgroup by the same
-// IEEE division instead of once per voxel), and the sinogram values.  Each
-// voxel then skips (ray, voxel) pairs that provably miss (the line passes
-// more than 0.02 voxel outside the box, where the float face tests of the
-// chord routine cannot succeed) and evaluates the chord of the rest with the
-// reference arithmetic, summing in the reference order.  Skipped pairs are
-// exact zeros (the reference adds l*y = +-0 for them, which leaves a sum
-// that starts at +0 unchanged), so the result is bit-identical to the
-// reference kernel.
+// ratios find_l_3d divides out (each the IEEE quotient, computed once per
+// workgroup instead of once per voxel), and the sinogram values.  Each voxel
+// then evaluates, with the reference arithmetic and in the reference order,
+// the chord of every cell in the intersection of the reference's window and
+// the voxel's shadow on the detector (widened by 0.05 cells).  Cells outside
+// the shadow have exact-zero chords, and the reference adds l*y = +-0 for
+// them, which leaves a sum that starts at +0 unchanged, so the result is
+// bit-identical to the reference kernel.  (Per-cell miss tests before the
+// chord were measured to cost more than the chords they saved.)
 namespace {
 constexpr int FP_A = 40;    // footprint columns held in LDS
 constexpr int FP_B = 32;    // footprint rows
-constexpr float MISS_MARGIN = 0.02f;
 
 __device__ __forceinline__ int wave_min(int v)
 {
@@ -59,14 +58,12 @@ __device__ __forceinline__ int wave_max(int v)
 // the order of accumulation are those of the reference kernel
 // (Atx_cone_mf_gpu_new.cu:58-86: views in order, rows outer, columns
 // inner), restricted to the cells whose chord can be non-zero.
-// Five workgroups per CU (the LDS tables take 27.7 KB each; the register
+// Five workgroups per CU (the LDS tables take 26.3 KB each; the register
 // budget that allows five is the measured optimum at 256^3).
 template <typename YT, typename XT, int KZ>
 __global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y, XT* __restrict__ X, AdjGeom g)
 {
     __shared__ float c_a[FP_A], c_b[FP_A], c_sba[FP_A], c_sab[FP_A];
-    __shared__ float c_nx[FP_A], c_ny[FP_A], c_off[FP_A], c_hs[FP_A];
-    __shared__ float c_ta[FP_A], c_tb[FP_A], c_t0[FP_A], c_hw[FP_A];
     __shared__ float c_ra[FP_A], c_rb[FP_A];
     __shared__ float r_c[FP_B], r_rc[FP_B];
     __shared__ float4 q_r[FP_A * FP_B];
@@ -168,13 +165,6 @@ __global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y
                 c_a[k] = a; c_b[k] = bb;
                 c_ra[k] = ra; c_rb[k] = rb;
                 c_sba[k] = fl_div_r(bb, a, ra); c_sab[k] = fl_div_r(a, bb, rb);
-                const float l2 = a * a + bb * bb, il = 1.0f / sqrtf(l2), il2 = 1.0f / l2;
-                c_nx[k] = bb * il; c_ny[k] = -a * il;
-                c_off[k] = bb * il * x1 - a * il * y1;
-                c_hs[k] = (fl_absf(a) + fl_absf(bb)) * 0.5f * il + MISS_MARGIN;
-                c_ta[k] = a * il2; c_tb[k] = bb * il2;
-                c_t0[k] = -(x1 * a + y1 * bb) * il2;
-                c_hw[k] = (fl_absf(a) + fl_absf(bb)) * 0.5f * il2;
             }
             for (int k = tid; k < nB; k += 256) {
                 const float cc = (g.zdet[B0 + k] + sz) - z1;
@@ -208,10 +198,6 @@ __global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y
                                 l = fl_chord_box3(x1, y1, z1, c * g.OD - s * yd, s * g.OD + c * yd,
                                                   g.zdet[ib] + sz, 1.0f, 1.0f, g.dz, xc, yc, zc[k]);
                             } else {
-                                const float dist = c_nx[ka] * xc + c_ny[ka] * yc - c_off[ka];
-                                if (fl_absf(dist) > c_hs[ka]) continue;
-                                const float tt = c_ta[ka] * xc + c_tb[ka] * yc + c_t0[ka];
-                                if (fl_absf(p[2] + cc * tt) > hz + fl_absf(cc) * c_hw[ka] + MISS_MARGIN) continue;
                                 const float4 r4 = q_r[kb * nA + ka];
                                 FlRatiosS q;
                                 q.r.sba = c_sba[ka]; q.r.sab = c_sab[ka];

@@ -98,16 +98,20 @@ def test_c3_gradient_full_size(gpu, oracle):
         worst = max(worst, abs(fd - g @ d.ravel().astype(np.float64)) / (np.linalg.norm(g) * np.linalg.norm(d)))
     _say(f"  FD error on the |g||d| scale: {worst:.2e}")
     assert worst <= 1e-4
-    # dot tests at full size
-    yv = torch.randn(P.sino_shape, device=gpu)
-    xv = torch.randn(P.vol_shape, device=gpu)
+    # dot tests at full size, seeded; uniform [0, 1) vectors keep <Ax, y>
+    # free of cancellation, so the relative difference measures the pair
+    gen = torch.Generator(device=gpu).manual_seed(13)
+    yv = torch.rand(P.sino_shape, device=gpu, generator=gen)
+    xv = torch.rand(P.vol_shape, device=gpu, generator=gen)
     lhs = float(torch.dot(P.forward(xv).double().ravel(), yv.double().ravel()))
     rhs = float(torch.dot(xv.double().ravel(), P.adjoint(yv, kind=ADJ_MATCHED).double().ravel()))
-    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+    _say(f"  projector dot test rel {abs(lhs - rhs) / abs(lhs):.2e}")
+    assert abs(lhs - rhs) <= 1e-6 * abs(lhs)
     c = W.forwardTransform(xv).frameletArray
-    cv = torch.randn_like(c)
+    cv = torch.rand(c.shape, device=gpu, generator=gen)
     lhs = float(torch.dot(c.double().ravel(), cv.double().ravel()))
     rhs = float(torch.dot(xv.double().ravel(), W.adjointTransform(cv).double().ravel()))
+    _say(f"  framelet dot test rel {abs(lhs - rhs) / abs(lhs):.2e}")
     assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
     # FP64 oracle gradient on every tenth view
     sub = G.cone_para(256, 256, 18, sd_phi=p.sd_phi[::10], **kw)
