@@ -9,6 +9,8 @@
 //   fan:  matlab/mex/src/Atx_fan_mf_cpu_new.cpp:35-62
 //   cone: matlab/mex/src/Atx_cone_mf_gpu_new.cu:41-84 (window from zr with
 //         d=sqrt((1+dz^2)/2), ceil upper bounds, box sides (1,1,dz)).
+#include <type_traits>
+
 #include "fl_chord.h"
 #include "fl_common.h"
 #include "fl_geom.h"
@@ -69,6 +71,7 @@ __global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y
     __shared__ float4 q_r[FP_A * FP_B];
     __shared__ float q_y[FP_A * FP_B];
     __shared__ int red[2][4][4];
+    __shared__ int s_special[2];   // per view (double-buffered): a footprint ray with no x or no y direction
 
     constexpr int BZ = 4 * KZ;
     const int bx = (g.nx + 7) / 8, by = (g.ny + 7) / 8, bz = (g.nz + BZ - 1) / BZ;
@@ -145,6 +148,8 @@ __global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y
                 red[j & 1][wv][0] = m0; red[j & 1][wv][1] = m1;
                 red[j & 1][wv][2] = m2; red[j & 1][wv][3] = m3;
             }
+            // last read two views ago, before the barrier every thread has since passed
+            if (tid == 0) s_special[j & 1] = 0;
         }
         __syncthreads();
         int A0 = red[j & 1][0][0], A1 = red[j & 1][0][1], B0 = red[j & 1][0][2], B1 = red[j & 1][0][3];
@@ -164,6 +169,7 @@ __global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y
                 const float ra = 1.0f / a, rb = 1.0f / bb;
                 c_a[k] = a; c_b[k] = bb;
                 c_ra[k] = ra; c_rb[k] = rb;
+                if (a == 0 || bb == 0) s_special[j & 1] = 1;
                 c_sba[k] = fl_div_r(bb, a, ra); c_sab[k] = fl_div_r(a, bb, rb);
             }
             for (int k = tid; k < nB; k += 256) {
@@ -181,7 +187,9 @@ __global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y
                 q_y[k] = fl_load(y, (size_t)(B0 + kb) * na + (A0 + ka));
             }
             __syncthreads();
-            if (alo <= ahi) {
+            // the chord's 2-D branch only where the view has such a ray:
+            // the common case runs without the per-cell test
+            auto cells = [&](auto special) {
                 const float h[3] = {0.5f, 0.5f, hz};
 #pragma unroll
                 for (int k = 0; k < KZ; k++) {
@@ -193,7 +201,7 @@ __global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y
                             const int ka = ia - A0;
                             const float a = c_a[ka], bb = c_b[ka];
                             float l;
-                            if (a == 0 || bb == 0) {
+                            if (decltype(special)::value && (a == 0 || bb == 0)) {
                                 const float yd = g.ydet[ia];
                                 l = fl_chord_box3(x1, y1, z1, c * g.OD - s * yd, s * g.OD + c * yd,
                                                   g.zdet[ib] + sz, 1.0f, 1.0f, g.dz, xc, yc, zc[k]);
@@ -211,6 +219,10 @@ __global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y
                         }
                     }
                 }
+            };
+            if (alo <= ahi) {
+                if (s_special[j & 1]) cells(std::true_type{});
+                else cells(std::false_type{});
             }
         } else if (alo <= ahi) {
 #pragma unroll
