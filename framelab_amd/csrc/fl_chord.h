@@ -219,21 +219,20 @@ FL_HD float fl_chord_box3_fast(const float* p, const float* h, bool xmajor, cons
     const bool fez = fl_absf(ez_u) <= h[0] && fl_absf(ez_v) <= h[1];
     const bool fxz = fl_absf(xz_u) <= h[0] && fl_absf(xz_v) <= h[1];
     const int ne = (int)fex + (int)fey + (int)fez, nx = (int)fxx + (int)fxy + (int)fxz;
-    if (ne == 1 && nx == 1) {
-        const float hx = h[0] * q.sg[0], hy = h[1] * q.sg[1], hz = h[2] * q.sg[2];
-        // entry point
-        const float e0 = fex ? -hx : (fey ? ey_u : ez_u);
-        const float e1 = fex ? ex_u : (fey ? -hy : ez_v);
-        const float e2 = fex ? ex_v : (fey ? ey_v : -hz);
-        // exit point
-        const float x0 = fxx ? hx : (fxy ? xy_u : xz_u);
-        const float x1 = fxx ? xx_u : (fxy ? hy : xz_v);
-        const float x2 = fxx ? xx_v : (fxy ? xy_v : hz);
-        const float u = x0 - e0, v = x1 - e1, w = x2 - e2;
-        return sqrtf(u * u + v * v + w * w);
-    }
-    if (ne + nx <= 1) return 0.0f;
-    return fl_chord_box3_r(p, h, xmajor, r);
+    // one entry and one exit candidate (the common case): the chord between
+    // them, formed without a branch (lanes of other cases discard it)
+    const float hx = h[0] * q.sg[0], hy = h[1] * q.sg[1], hz = h[2] * q.sg[2];
+    const float e0 = fex ? -hx : (fey ? ey_u : ez_u);
+    const float e1 = fex ? ex_u : (fey ? -hy : ez_v);
+    const float e2 = fex ? ex_v : (fey ? ey_v : -hz);
+    const float x0 = fxx ? hx : (fxy ? xy_u : xz_u);
+    const float x1 = fxx ? xx_u : (fxy ? hy : xz_v);
+    const float x2 = fxx ? xx_v : (fxy ? xy_v : hz);
+    const float u = x0 - e0, v = x1 - e1, w = x2 - e2;
+    const bool one = ne == 1 && nx == 1;
+    float l = one ? sqrtf(u * u + v * v + w * w) : 0.0f;
+    if (!one && ne + nx > 1) l = fl_chord_box3_r(p, h, xmajor, r);   // edges and corners
+    return l;
 }
 
 // Detector cells whose ray can meet a voxel box, from the projection of the
