@@ -41,16 +41,10 @@ namespace {
 constexpr int FP_A = 40;    // footprint columns held in LDS
 constexpr int FP_B = 32;    // footprint rows
 
-__device__ __forceinline__ int wave_min(int v)
-{
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ int wave_max(int v)
-{
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
-}
+// wave-wide min / max (DPP row shifts and lane reads, no LDS round trips);
+// called with every lane of the wave active
+__device__ __forceinline__ int wave_min(int v) { return __ockl_wfred_min_i32(v); }
+__device__ __forceinline__ int wave_max(int v) { return __ockl_wfred_max_i32(v); }
 }  // namespace
 
 // Exact cone adjoint, z-column form.  A workgroup is an 8 x 8 x (4*KZ)

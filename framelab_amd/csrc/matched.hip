@@ -45,16 +45,10 @@ namespace {
 constexpr int MF_A = 48;   // footprint columns in LDS
 constexpr int MF_B = 32;   // footprint rows
 
-__device__ __forceinline__ int mwave_min(int v)
-{
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ int mwave_max(int v)
-{
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
-}
+// wave-wide min / max (DPP row shifts and lane reads, no LDS round trips);
+// called with every lane of the wave active
+__device__ __forceinline__ int mwave_min(int v) { return __ockl_wfred_min_i32(v); }
+__device__ __forceinline__ int mwave_max(int v) { return __ockl_wfred_max_i32(v); }
 }  // namespace
 
 // 3D, footprint-staged, z-column form: a workgroup is an 8 x 8 x (4*KZ)
