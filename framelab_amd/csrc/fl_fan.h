@@ -136,7 +136,10 @@ __device__ __forceinline__ float fl_fan_adj_pixel(const YT* __restrict__ Y, cons
             const float Py = cd * c - sd * s;
             const float Pd = g.SO * sd;
             const float l = fl_chord_line2(Px, Py, Pd, xc, yc);
-            if (l > 0) acc = acc + l * fl_load(y, (size_t)id);
+            // a select, not a branch around the load: lanes of a wave
+            // disagree on l > 0 too often for a branch to pay
+            const float yv = fl_load(y, (size_t)id);
+            acc = l > 0 ? acc + l * yv : acc;
         }
     }
     return acc * g.scale;
