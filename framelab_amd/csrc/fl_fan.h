@@ -166,6 +166,23 @@ __device__ __forceinline__ FlAxisIv fl_slab(const SidAxis& a, int cell)
     return r;
 }
 
+// As fl_slab, given rd = 1.0f / a.d: both quotients from the one reciprocal
+// (fl_div_r, bit-identical to the divisions).
+__device__ __forceinline__ FlAxisIv fl_slab_r(const SidAxis& a, int cell, float rd)
+{
+    FlAxisIv r;
+    if (!a.on) {
+        r.in = -FL_INF;
+        r.out = FL_INF;
+        return r;
+    }
+    const float lo = fl_div_r((float)(cell - a.nh) * a.s - a.o, a.d, rd);
+    const float hi = fl_div_r((float)(cell - a.nh + 1) * a.s - a.o, a.d, rd);
+    r.in = a.st > 0 ? lo : hi;
+    r.out = a.st > 0 ? hi : lo;
+    return r;
+}
+
 // Matched (Siddon-transpose) adjoint of pixel (ix, iy) of frame it, scaled.
 template <typename YT>
 __device__ __forceinline__ float fl_fan_matched_pixel(const YT* __restrict__ Y, const FwdGeom& f, const AdjGeom& g,
@@ -198,7 +215,7 @@ __device__ __forceinline__ float fl_fan_matched_pixel(const YT* __restrict__ Y, 
             SidAxis ax, ay;
             sid_axis_init(ax, x1, dx, 1.0f, f.nx);
             sid_axis_init(ay, y1, dy, 1.0f, f.ny);
-            const FlAxisIv sx = fl_slab(ax, ix), sy = fl_slab(ay, iy);
+            const FlAxisIv sx = fl_slab_r(ax, ix, 1.0f / dx), sy = fl_slab_r(ay, iy, 1.0f / dy);
             const float ain = fmaxf(sx.in, sy.in), aout = fminf(sx.out, sy.out);
             if (!(aout > ain)) continue;
             const float m = (ain + aout) * 0.5f;

@@ -209,14 +209,14 @@ __global__ __launch_bounds__(256, 6) void k_adj_matched_cone(const YT* __restric
                     const float dz = (f.zdet[ib] + sz) - z1;
                     SidAxis az;
                     sid_axis_init(az, z1, dz, f.dz0, f.nz);
-                    const FlAxisIv sv = fl_slab(az, iz);
+                    const FlAxisIv sv = fl_slab_r(az, iz, 1.0f / dz);
                     for (int ia = alo; ia <= ahi; ia++) {
                         const float yd = f.ydet[ia];
                         const float dx = (c * f.OD - s * yd) - x1, dy = (s * f.OD + c * yd) - y1;
                         SidAxis ax, ay;
                         sid_axis_init(ax, x1, dx, 1.0f, f.nx);
                         sid_axis_init(ay, y1, dy, 1.0f, f.ny);
-                        const FlAxisIv sx = fl_slab(ax, ix), sy = fl_slab(ay, iy);
+                        const FlAxisIv sx = fl_slab_r(ax, ix, 1.0f / dx), sy = fl_slab_r(ay, iy, 1.0f / dy);
                         const float L = sqrtf(dx * dx + dy * dy + dz * dz);
                         const float Si = matched_coef(sx.in, sx.out, sy.in, sy.out, sv.in, sv.out, ax, ay, az, ix,
                                                       iy, iz, dz_one, L);
