@@ -241,6 +241,31 @@ __global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y
     }
 }
 
+// Per (view, cell) of a fan plan, the two quotients of the cell's line
+// chord that do not depend on the pixel (fl_chord_line2_q): the line's
+// coefficients formed exactly as fl_fan_adj_pixel forms them.
+__global__ __launch_bounds__(256) void k_fan_line_q(AdjGeom g, int nv, float2* __restrict__ out)
+{
+    const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= (size_t)nv * g.na) return;
+    const int iv = (int)(k / g.na), id = (int)(k % g.na);
+    const float c = g.cphi[iv], s = g.sphi[iv];
+    const float sd = g.sdet[id], cd = g.cdet[id];
+    const float Px = -(sd * c + cd * s);
+    const float Py = cd * c - sd * s;
+    const float a2 = fl_absf(Px), b2 = fl_absf(Py);
+    const float hx = a2 / 2, hy = b2 / 2;
+    out[k] = make_float2((hx < hy) ? 1.0f / b2 : 1.0f / a2, 1.0f / (a2 * b2));
+}
+
+hipError_t fl_launch_fan_line_q(const AdjGeom& g, int nv, float2* out, hipStream_t st)
+{
+    const size_t n = (size_t)nv * g.na;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fan_line_q, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, nv, out);
+    return hipGetLastError();
+}
+
 template <typename YT, typename XT>
 static hipError_t launch_adj_fan_t(const void* Y, void* X, const AdjGeom& g, hipStream_t st)
 {

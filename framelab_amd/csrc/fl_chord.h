@@ -38,6 +38,23 @@ FL_HD float fl_chord_line2(float a, float b, float c, float x, float y)
     return l;
 }
 
+// fl_chord_line2 with the line's own quotients supplied, q = (1/max(|a|,
+// |b|), RN(1/(|a||b|))) (the same for every pixel a line meets): the flat
+// value is q.x itself, and (dmax - d)/(|a||b|) is formed from q.y by the
+// Markstein correction, bit-identical to the division.
+__device__ __forceinline__ float fl_chord_line2_q(float a, float b, float c, float x, float y, float2 q)
+{
+    float a2 = fl_absf(a), b2 = fl_absf(b);
+    float hx = a2 / 2, hy = b2 / 2;
+    float dmax = hx + hy;
+    float t = c - a * x - b * y;
+    float d = fl_absf(t);
+    float d0 = fl_absf(hx - hy);
+    const float slope = fl_div_r(dmax - d, a2 * b2, q.y);
+    const float l = (d <= d0) ? q.x : slope;
+    return d < dmax ? l : 0.0f;
+}
+
 FL_HD float fl_chord_seg2(float x1, float y1, float x2, float y2, float dx, float dy, float x, float y)
 {
     float a = x2 - x1, b = y2 - y1;

@@ -110,7 +110,10 @@ __device__ __forceinline__ float fl_fan_gao_sum(const VT* __restrict__ x, const 
 
 // Exact pixel-driven adjoint of pixel (ix, iy) of frame it, scaled
 // (Atx_fan_mf_cpu_new.cpp:35-62).
-template <typename YT>
+// LQ: take the lines' pixel-independent quotients from g.line_q (one table
+// read per cell instead of up to three divisions); the batched kernel,
+// whose small persistent workgroups hide the extra read worse, divides.
+template <typename YT, bool LQ = true>
 __device__ __forceinline__ float fl_fan_adj_pixel(const YT* __restrict__ Y, const AdjGeom& g, int it, int ix, int iy)
 {
     const int nd = g.na, nd2 = nd / 2;
@@ -130,12 +133,13 @@ __device__ __forceinline__ float fl_fan_adj_pixel(const YT* __restrict__ Y, cons
         int lo = (int)floorf((yr - 1) * t - g.y_os + (float)nd2);
         if (lo < 0) lo = 0;
         if (hi > nd - 1) hi = nd - 1;
+        const float2* __restrict__ lq = LQ ? g.line_q + (size_t)iv * nd : nullptr;
         for (int id = lo; id <= hi; id++) {
             const float sd = g.sdet[id], cd = g.cdet[id];
             const float Px = -(sd * c + cd * s);
             const float Py = cd * c - sd * s;
             const float Pd = g.SO * sd;
-            const float l = fl_chord_line2(Px, Py, Pd, xc, yc);
+            const float l = LQ ? fl_chord_line2_q(Px, Py, Pd, xc, yc, lq[id]) : fl_chord_line2(Px, Py, Pd, xc, yc);
             // a select, not a branch around the load: lanes of a wave
             // disagree on l > 0 too often for a branch to pay
             const float yv = fl_load(y, (size_t)id);

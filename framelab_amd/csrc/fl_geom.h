@@ -37,4 +37,7 @@ struct AdjGeom {
     size_t nvox;
     size_t nd;              // detector cells per view
     float ya0, inv_pa, zb0, inv_pb;
+    // fan: per (view, cell) the view-only quotients of the line chord,
+    // (1/max(|a|,|b|), 1/(|a||b|)), at [iv*nd + id]
+    const float2* line_q;
 };
