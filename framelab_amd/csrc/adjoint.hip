@@ -258,6 +258,31 @@ __global__ __launch_bounds__(256) void k_fan_line_q(AdjGeom g, int nv, float2* _
     out[k] = make_float2((hx < hy) ? 1.0f / b2 : 1.0f / a2, 1.0f / (a2 * b2));
 }
 
+// Per (view, cell) of a fan plan: (1/dx, 1/dy, |d|) of the ray, formed as
+// fl_fan_matched_pixel forms them (zero direction components give infinite
+// reciprocals, which the slab routine never uses).
+__global__ __launch_bounds__(256) void k_fan_ray_q(FwdGeom f, int nv, float4* __restrict__ out)
+{
+    const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= (size_t)nv * f.na) return;
+    const int iv = (int)(k / f.na), id = (int)(k % f.na);
+    const float c = f.cs[iv], s = f.sn[iv];
+    const float x1 = c * (-f.SO), y1 = s * (-f.SO);
+    const float yd = f.ydet[id];
+    const float x2 = c * f.OD - s * yd;
+    const float y2 = s * f.OD + c * yd;
+    const float dx = x2 - x1, dy = y2 - y1;
+    out[k] = make_float4(1.0f / dx, 1.0f / dy, sqrtf(dx * dx + dy * dy), 0.0f);
+}
+
+hipError_t fl_launch_fan_ray_q(const FwdGeom& f, int nv, float4* out, hipStream_t st)
+{
+    const size_t n = (size_t)nv * f.na;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fan_ray_q, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, f, nv, out);
+    return hipGetLastError();
+}
+
 hipError_t fl_launch_fan_line_q(const AdjGeom& g, int nv, float2* out, hipStream_t st)
 {
     const size_t n = (size_t)nv * g.na;

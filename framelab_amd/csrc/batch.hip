@@ -91,7 +91,7 @@ __global__ __launch_bounds__(64) void k_badj(const float* __restrict__ Y, float*
         const int ix = (local % tx) * 8 + (threadIdx.x & 7), iy = (local / tx) * 8 + (threadIdx.x >> 3);
         if (ix < nx && iy < ny) {
             const float* y = Y + D[p].soff;
-            const float v = MATCHED ? fl_fan_matched_pixel(y, D[p].f, D[p].a, 0, ix, iy)
+            const float v = MATCHED ? fl_fan_matched_pixel<float, false>(y, D[p].f, D[p].a, 0, ix, iy)
                                     : fl_fan_adj_pixel<float, false>(y, D[p].a, 0, ix, iy);
             X[D[p].voff + (size_t)iy * nx + ix] = v;
         }

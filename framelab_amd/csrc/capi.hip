@@ -72,7 +72,8 @@ static void plan_free_arrays(fl_plan_s* p)
 {
     void* ptrs[] = {p->d_cs, p->d_sn, p->d_sdz, p->d_ydet, p->d_zdet, p->d_cphi, p->d_sphi,
                     p->d_cdet, p->d_sdet, p->d_slot_view, p->d_view_frame, p->d_idY, p->d_Nv, p->d_bricks,
-                    p->d_fb_px, p->d_fb_py, p->d_fb_pd, p->d_cdet2, p->d_sdet2, p->d_line_q};
+                    p->d_fb_px, p->d_fb_py, p->d_fb_pd, p->d_cdet2, p->d_sdet2, p->d_line_q,
+                    p->d_ray_q};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     if (p->fwd_done) (void)hipEventDestroy(p->fwd_done);
@@ -288,7 +289,9 @@ extern "C" int fl_plan_create(fl_plan* out, const fl_geometry* g)
     if (!cone) {
         // the fan adjoint's per-line quotients, built once on the device
         hipError_t e = hipMalloc((void**)&p->d_line_q, (size_t)g->nv * g->na * sizeof(float2));
+        if (e == hipSuccess) e = hipMalloc((void**)&p->d_ray_q, (size_t)g->nv * g->na * sizeof(float4));
         if (e == hipSuccess) e = fl_launch_fan_line_q(p->ag, g->nv, p->d_line_q, nullptr);
+        if (e == hipSuccess) e = fl_launch_fan_ray_q(p->fg, g->nv, p->d_ray_q, nullptr);
         if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
         if (e != hipSuccess) {
             plan_free_arrays(p);
@@ -296,6 +299,7 @@ extern "C" int fl_plan_create(fl_plan* out, const fl_geometry* g)
             return fl_hip_fail(e, "fan line quotients");
         }
         a.line_q = p->d_line_q;
+        a.ray_q = p->d_ray_q;
     }
     if (cone && !(g->dy_det > 0 && g->dz_det > 0)) {
         plan_free_arrays(p);

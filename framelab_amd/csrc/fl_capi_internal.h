@@ -31,6 +31,7 @@ struct fl_plan_s {
     float *d_fb_px = nullptr, *d_fb_py = nullptr, *d_fb_pd = nullptr;
     float *d_cdet2 = nullptr, *d_sdet2 = nullptr;
     float2* d_line_q = nullptr;   // fan: per (view, cell) line-chord quotients
+    float4* d_ray_q = nullptr;    // fan: per (view, cell) ray reciprocals and length
     std::vector<float> h_ydet;
     // largest distance, in cells, of a detector coordinate from the uniform
     // grid through the first and last cell: the adjoints' window maps assume
@@ -59,6 +60,7 @@ hipError_t fl_launch_trace(const FwdGeom& g, int cone, int nrays, const int* ray
 hipError_t fl_launch_ynorm(const float* y, float* yn, const FwdGeom& g, hipStream_t st);
 hipError_t fl_launch_adj_fan(const void* Y, int yt, void* X, int xt, const AdjGeom& g, hipStream_t st);
 hipError_t fl_launch_fan_line_q(const AdjGeom& g, int nv, float2* out, hipStream_t st);
+hipError_t fl_launch_fan_ray_q(const FwdGeom& f, int nv, float4* out, hipStream_t st);
 hipError_t fl_launch_adj_cone(const void* Y, int yt, void* X, int xt, const AdjGeom& g, hipStream_t st);
 hipError_t fl_launch_adj_matched(const void* Y, int yt, void* X, int xt, const fl_plan_s* p, hipStream_t st);
 hipError_t fl_launch_fwd_fan_fb(const float* X, float* Y, const fl_plan_s* p, hipStream_t st);

@@ -188,7 +188,9 @@ __device__ __forceinline__ FlAxisIv fl_slab_r(const SidAxis& a, int cell, float 
 }
 
 // Matched (Siddon-transpose) adjoint of pixel (ix, iy) of frame it, scaled.
-template <typename YT>
+// RQ: take each ray's reciprocals and length from g.ray_q instead of
+// forming them per pixel (same values, the same IEEE operations made once).
+template <typename YT, bool RQ = true>
 __device__ __forceinline__ float fl_fan_matched_pixel(const YT* __restrict__ Y, const FwdGeom& f, const AdjGeom& g,
                                                       int it, int ix, int iy)
 {
@@ -211,20 +213,22 @@ __device__ __forceinline__ float fl_fan_matched_pixel(const YT* __restrict__ Y, 
         if (lo < 0) lo = 0;
         if (hi > nd - 1) hi = nd - 1;
         const float x1 = c * (-f.SO), y1 = s * (-f.SO);
+        const float4* __restrict__ rq = RQ ? g.ray_q + (size_t)iv * nd : nullptr;
         for (int id = lo; id <= hi; id++) {
             const float yd = f.ydet[id];
             const float x2 = c * f.OD - s * yd;
             const float y2 = s * f.OD + c * yd;
             const float dx = x2 - x1, dy = y2 - y1;
+            const float4 q = RQ ? rq[id] : make_float4(1.0f / dx, 1.0f / dy, 0.0f, 0.0f);
             SidAxis ax, ay;
             sid_axis_init(ax, x1, dx, 1.0f, f.nx);
             sid_axis_init(ay, y1, dy, 1.0f, f.ny);
-            const FlAxisIv sx = fl_slab_r(ax, ix, 1.0f / dx), sy = fl_slab_r(ay, iy, 1.0f / dy);
+            const FlAxisIv sx = fl_slab_r(ax, ix, q.x), sy = fl_slab_r(ay, iy, q.y);
             const float ain = fmaxf(sx.in, sy.in), aout = fminf(sx.out, sy.out);
             if (!(aout > ain)) continue;
             const float m = (ain + aout) * 0.5f;
             if (sid_cell_xy(ax, m) != ix || sid_cell_xy(ay, m) != iy) continue;
-            const float L = sqrtf(dx * dx + dy * dy);
+            const float L = RQ ? q.z : sqrtf(dx * dx + dy * dy);
             acc = acc + ((aout - ain) * L) * fl_load(y, (size_t)id);
         }
     }

@@ -40,4 +40,6 @@ struct AdjGeom {
     // fan: per (view, cell) the view-only quotients of the line chord,
     // (1/max(|a|,|b|), 1/(|a||b|)), at [iv*nd + id]
     const float2* line_q;
+    // fan: per (view, cell) (1/dx, 1/dy, |d|) of the ray, for the matched adjoint
+    const float4* ray_q;
 };
