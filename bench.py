@@ -266,7 +266,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); multi-GPU sharding is "
-              "described, not run, in this build", file=sys.stderr)
+              "described, not run, here", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -1,9 +1,10 @@
 /*
  * ct_oracle.c -- CPU restatement of FrameLab's CT projector arithmetic.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP product
- * path: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
- * may load the library built from it.  Nothing in framelab_amd/ links it.
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle the HIP product path is
+ * compared with: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load the library built from it.  Nothing in
+ * framelab_amd/ links it.
  *
  * Every routine restates the reference algorithm in FP32 with the reference's
  * operation order (no FMA contraction: built with -ffp-contract=off), so that

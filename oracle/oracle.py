@@ -1,8 +1,9 @@
 """CPU oracle for the CT-operator hot path.
 
 TEST INFRASTRUCTURE ONLY.  Imported by tests/, ``__graft_entry__.smoke()`` and
-``bench.py``'s cpu_baseline leg, and only as the checker: nothing in the
-product package (framelab_amd/) imports it.
+``bench.py``'s cpu_baseline leg, and only as the reference results the
+GPU path is compared with: nothing in the product package (framelab_amd/)
+imports it.
 
 Contents
   * ctypes bindings to liboracle.so (ct_oracle.c, this repository's C
