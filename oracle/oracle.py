@@ -14,7 +14,12 @@ Contents
     and split Bregman (matlab/classes/+Transforms, +Optimizers, +DataTypes).
 
 Parity pinning per function is listed in each docstring; functions with no
-independent check say "parity unpinned".
+independent check say "parity unpinned".  Parity unpinned against the
+reference (no reference values exist for them: MATLAB is absent and no
+reference test holds any): the framelet band values (fradec*/frarec*; pinned
+only by scipy correlation, W^T W = I and the tight-frame identities), the
+CGLS iterates (cgls; pinned by the dense normal-equation solution) and split
+Bregman (split_bregman; pinned by a fixed point and objective decrease).
 """
 from __future__ import annotations
 
@@ -446,7 +451,9 @@ AX_X2, AX_Y2 = 1, 0
 
 def fradec2d(A, D, level=1):
     """FraDecMultiLevel2D.m:16-53: band(i,j) = corr_y[D_j](corr_x[D_i](A)).
-    Returns a list over levels of arrays [nD, nD, ny, nx]."""
+    Returns a list over levels of arrays [nD, nD, ny, nx].  Band values:
+    parity unpinned against the reference (pinned by scipy correlation and
+    the tight-frame identities)."""
     out = []
     cur = np.asarray(A, dtype=np.float64)
     n = len(D)
@@ -539,7 +546,9 @@ def soft_threshold(C, tau, dim):
 # --------------------------------------------------------------------------
 
 def cgls(A, At, b, u0, lam, iters, tol=0.0, log=None):
-    """CG on (At A + lam I) u = b, solveCGLS.m:1-31.  A, At are callables."""
+    """CG on (At A + lam I) u = b, solveCGLS.m:1-31.  A, At are callables.
+    Iterates: parity unpinned against the reference (pinned by the dense
+    solution of the regularised normal equations)."""
     Aloc = lambda v: At(A(v)) + lam * v
     r = Aloc(u0) - b
     p = -r
