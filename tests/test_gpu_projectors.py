@@ -223,9 +223,14 @@ def test_nonuniform_detector(gpu, oracle):
             P.adjoint(yt) if kind is None else P.adjoint(yt, kind=kind)
 
 
-def test_lowp_storage(gpu):
-    _say("case: bf16/fp16 storage forward + adjoint")
-    p = small_cone()
+@pytest.mark.parametrize("geom", ["cone", "fan"])
+def test_lowp_storage(gpu, geom):
+    """BF16/FP16 storage of the volume and the sinogram (FP32 arithmetic):
+    forward, exact and matched adjoints within the storage rounding of the
+    FP32-storage results."""
+    from framelab_amd.projector import ADJ_EXACT, ADJ_MATCHED
+    _say(f"case: bf16/fp16 storage forward + adjoints ({geom})")
+    p = small_cone() if geom == "cone" else small_fan()
     P = _proj(p)
     g = torch.Generator(device="cpu").manual_seed(9)
     x = torch.rand(P.vol_shape, generator=g).to(gpu)
@@ -235,10 +240,15 @@ def test_lowp_storage(gpu):
         assert y.dtype == dt
         rel = ((y.float() - y32).norm() / y32.norm()).item()
         assert rel < tol, (dt, rel)
-        xb = P.adjoint(y32.to(dt), dtype=torch.float32)
-        xr = P.adjoint(y32)
-        rel = ((xb - xr).norm() / xr.norm()).item()
-        assert rel < tol, (dt, rel)
+        for kind in (ADJ_EXACT, ADJ_MATCHED):
+            xb = P.adjoint(y32.to(dt), dtype=torch.float32, kind=kind)
+            xr = P.adjoint(y32, kind=kind)
+            rel = ((xb - xr).norm() / xr.norm()).item()
+            assert rel < tol, (dt, kind, rel)
+            xl = P.adjoint(y32.to(dt), kind=kind)
+            assert xl.dtype == dt
+            rel = ((xl.float() - xr).norm() / xr.norm()).item()
+            assert rel < tol, (dt, kind, rel)
 
 
 @pytest.mark.parametrize("shape", [(96, 40, 48, 48), (64, 36, 32, 40)])
