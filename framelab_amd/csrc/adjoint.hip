@@ -32,10 +32,12 @@ __global__ __launch_bounds__(256) void k_adj_fan(const YT* __restrict__ Y, XT* _
 // workgroup instead of once per voxel), and the sinogram values.  Each voxel
 // then evaluates, with the reference arithmetic and in the reference order,
 // the chord of every cell in the intersection of the reference's window and
-// the voxel's shadow on the detector (widened by 0.05 cells).  Cells outside
-// the shadow have exact-zero chords, and the reference adds l*y = +-0 for
-// them, which leaves a sum that starts at +0 unchanged, so the result is
-// bit-identical to the reference kernel.  (Per-cell miss tests before the
+// the voxel's shadow on the detector (widened by 0.05 cells), or of the
+// shadow alone where the plan has shown that the reference's window contains
+// it (AdjGeom::win_ref clear).  Cells outside the shadow have exact-zero
+// chords, and the reference adds l*y = +-0 for them, which leaves a sum that
+// starts at +0 unchanged, so the result is bit-identical to the reference
+// kernel.  (Per-cell miss tests before the
 // chord were measured to cost more than the chords they saved.)
 namespace {
 constexpr int FP_A = 40;    // footprint columns held in LDS
@@ -104,27 +106,30 @@ __global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y
         if (col) {
             const float xr = c * xc + s * yc;
             const float yr = -s * xc + c * yc;
-            float t = SD / ((xr + g.SO) * g.dy_det);
-            ahi = (int)ceilf((yr + 1) * t - g.y_os + (float)na2);
-            alo = (int)floorf((yr - 1) * t - g.y_os + (float)na2);
-            if (alo < 0) alo = 0;
-            if (ahi > na - 1) ahi = na - 1;
+            // shadow window (clamped to the panel), cut by the reference's
+            // window where that can bind (the clamps of the latter are
+            // implied by the former's)
             const ShadowCols sc = fl_shadow_cols(c, s, g.SO, SD, xc, yc, g.ya0, g.inv_pa, 0.05f, na);
-            alo = max(alo, sc.alo);
-            ahi = min(ahi, sc.ahi);
-            t = SD / ((xr + g.SO) * g.dz_det);
+            alo = sc.alo;
+            ahi = sc.ahi;
+            if (g.win_ref & 1) {
+                const float t = SD / ((xr + g.SO) * g.dy_det);
+                ahi = min(ahi, (int)ceilf((yr + 1) * t - g.y_os + (float)na2));
+                alo = max(alo, (int)floorf((yr - 1) * t - g.y_os + (float)na2));
+            }
+            const bool rows_ref = (g.win_ref & 2) != 0;
+            float t = 0.0f;
+            if (rows_ref) t = SD / ((xr + g.SO) * g.dz_det);
 #pragma unroll
             for (int k = 0; k < KZ; k++) {
                 if (iz0 + k < g.nz && alo <= ahi) {
                     const float zr = zc[k] - sz;
-                    int bh = (int)ceilf((zr + d) * t + (float)nb2);
-                    int bl = (int)floorf((zr - d) * t + (float)nb2);
-                    if (bl < 0) bl = 0;
-                    if (bh > nb - 1) bh = nb - 1;
-                    int sl, sh;
-                    fl_shadow_rows(sc, zr, hz, g.zb0, g.inv_pb, 0.05f, nb, sl, sh);
-                    bl = max(bl, sl);
-                    bh = min(bh, sh);
+                    int bl, bh;
+                    fl_shadow_rows(sc, zr, hz, g.zb0, g.inv_pb, 0.05f, nb, bl, bh);
+                    if (rows_ref) {
+                        bh = min(bh, (int)ceilf((zr + d) * t + (float)nb2));
+                        bl = max(bl, (int)floorf((zr - d) * t + (float)nb2));
+                    }
                     if (bl <= bh) {
                         blo[k] = bl;
                         bhi[k] = bh;

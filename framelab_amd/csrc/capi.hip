@@ -106,6 +106,48 @@ static int fwd_order_end(fl_plan_s* p, hipStream_t st, bool order)
     return FL_OK;
 }
 
+// AdjGeom::win_ref for a cone plan: whether the reference's detector window
+// (Atx_cone_mf_gpu_new.cu:64-73) can exclude a cell of a voxel's shadow
+// window, per axis (bit 0 columns, bit 1 rows).  The rotated unit square of
+// a voxel reaches ex = (|cos|+|sin|)/2 from its centre both along and across
+// the ray axis, so at centre depth D = SO + xr its shadow's outer column
+// edge is at most (|yr| + ex) SD / (D - ex) from the axis, the window's at
+// (|yr| + 1) SD / D; the first is the smaller when ex (|yr| + 1) <= (1 - ex) D.
+// For rows (half-height dz/2 against the window's d = sqrt((1+dz^2)/2)) the
+// condition is ex (|zr| + d) <= (d - dz/2) D.  Both are checked at their
+// worst voxel and view.  The two windows' index maps then differ by a
+// constant (half a cell and y_os on the reference grid), which must leave
+// 0.15 cells for the shadow's 0.05-cell margin and the rounding of both.
+static int cone_ref_window_bits(const fl_geometry* g, double pa, double pb, const std::vector<float>& cphi,
+                                const std::vector<float>& sphi, const std::vector<float>& sdz)
+{
+    double ex = 0.0;
+    for (size_t v = 0; v < cphi.size(); v++) ex = fmax(ex, 0.5 * (fabs((double)cphi[v]) + fabs((double)sphi[v])));
+    ex += 1e-6;
+    const double SO = g->SO, SD = (double)g->SO + g->OD;
+    const double rc = 0.5 * sqrt((double)(g->nx - 1) * (g->nx - 1) + (double)(g->ny - 1) * (g->ny - 1));
+    const double dmin = SO - rc - 1.0;   // below every voxel centre's depth
+    if (!(dmin > 0) || !(ex < 1)) return 3;
+    int bits = 0;
+    {
+        const double geo = (1 - ex) * SO - ex - rc * sqrt((1 - ex) * (1 - ex) + ex * ex);
+        const double slack = fabs(1.0 / pa - 1.0 / g->dy_det) * (rc + 1) * SD / dmin;
+        const double off = -(double)g->y_det[0] / pa + g->y_os - (double)(g->na / 2);
+        if (!(pa > 0 && g->dy_det > 0 && geo > 1e-3 * SO && fabs(off) + slack < 0.8)) bits |= 1;
+    }
+    {
+        const double dz = g->dz, hz = dz / 2, d = sqrt((1 + dz * dz) / 2);
+        const double zlo = (0.5 - g->nz / 2) * dz, zhi = (g->nz - 0.5 - g->nz / 2) * dz;
+        double zmax = 0.0;
+        for (float s : sdz) zmax = fmax(zmax, fmax(fabs(zlo - s), fabs(zhi - s)));
+        const double geo = (d - hz) * dmin - ex * (zmax + d);
+        const double slack = fabs(1.0 / pb - 1.0 / g->dz_det) * (zmax + d) * SD / dmin;
+        const double off = -(double)g->z_det[0] / pb - (double)(g->nb / 2);
+        if (!(pb > 0 && g->dz_det > 0 && geo > 1e-3 * SO && fabs(off) + slack < 0.8)) bits |= 2;
+    }
+    return bits;
+}
+
 extern "C" int fl_plan_create(fl_plan* out, const fl_geometry* g)
 {
     if (!out || !g) return fl_fail(FL_ERR_INVALID, "null plan or geometry");
@@ -277,6 +319,7 @@ extern "C" int fl_plan_create(fl_plan* out, const fl_geometry* g)
         for (int k = 0; cone && k < g->nb && pb != 0.0; k++)
             dev = fmax(dev, fabs(((double)g->z_det[k] - g->z_det[0]) / pb - k));
         p->det_dev = dev;
+        a.win_ref = cone ? cone_ref_window_bits(g, pa, pb, cphi, sphi, sdz) : 3;
         const double rvol = 0.5 * sqrt((double)g->nx * g->nx + (double)g->ny * g->ny) + 1.0;
         if (!(pa != 0.0 && pb != 0.0) || !(g->SO > rvol)) {
             plan_free_arrays(p);

@@ -216,15 +216,18 @@ struct FlRatiosS {
 FL_HD float fl_chord_box3_fast(const float* p, const float* h, bool xmajor, const FlRatiosS& q)
 {
     const FlRatios& r = q.r;
+    // signed half-sides: r*(h*sg) rounds to (r*h)*sg exactly, since the
+    // factor sg = +-1 only flips the sign and rounding is sign-symmetric
+    const float hx = h[0] * q.sg[0], hy = h[1] * q.sg[1], hz = h[2] * q.sg[2];
     // x faces: U = y (slope sba), V = z (slope sca)
-    const float mxu = r.sba * (-p[0]) + p[1], wxu = r.sba * h[0] * q.sg[0];
-    const float mxv = r.sca * (-p[0]) + p[2], wxv = r.sca * h[0] * q.sg[0];
+    const float mxu = r.sba * (-p[0]) + p[1], wxu = r.sba * hx;
+    const float mxv = r.sca * (-p[0]) + p[2], wxv = r.sca * hx;
     // y faces: U = x (slope sab), V = z (slope scb)
-    const float myu = r.sab * (-p[1]) + p[0], wyu = r.sab * h[1] * q.sg[1];
-    const float myv = r.scb * (-p[1]) + p[2], wyv = r.scb * h[1] * q.sg[1];
+    const float myu = r.sab * (-p[1]) + p[0], wyu = r.sab * hy;
+    const float myv = r.scb * (-p[1]) + p[2], wyv = r.scb * hy;
     // z faces: U = x (slope sac), V = y (slope sbc)
-    const float mzu = r.sac * (-p[2]) + p[0], wzu = r.sac * h[2] * q.sg[2];
-    const float mzv = r.sbc * (-p[2]) + p[1], wzv = r.sbc * h[2] * q.sg[2];
+    const float mzu = r.sac * (-p[2]) + p[0], wzu = r.sac * hz;
+    const float mzv = r.sbc * (-p[2]) + p[1], wzv = r.sbc * hz;
     // entry (e) and exit (x) candidates
     const float ex_u = mxu - wxu, ex_v = mxv - wxv, xx_u = mxu + wxu, xx_v = mxv + wxv;
     const float ey_u = myu - wyu, ey_v = myv - wyv, xy_u = myu + wyu, xy_v = myv + wyv;
@@ -238,7 +241,6 @@ FL_HD float fl_chord_box3_fast(const float* p, const float* h, bool xmajor, cons
     const int ne = (int)fex + (int)fey + (int)fez, nx = (int)fxx + (int)fxy + (int)fxz;
     // one entry and one exit candidate (the common case): the chord between
     // them, formed without a branch (lanes of other cases discard it)
-    const float hx = h[0] * q.sg[0], hy = h[1] * q.sg[1], hz = h[2] * q.sg[2];
     const float e0 = fex ? -hx : (fey ? ey_u : ez_u);
     const float e1 = fex ? ex_u : (fey ? -hy : ez_v);
     const float e2 = fex ? ex_v : (fey ? ey_v : -hz);

@@ -37,6 +37,10 @@ struct AdjGeom {
     size_t nvox;
     size_t nd;              // detector cells per view
     float ya0, inv_pa, zb0, inv_pb;
+    // cone exact adjoint: bit 0 (columns) / bit 1 (rows) set where the
+    // reference's detector window may cut a voxel's shadow window and has
+    // to be applied; clear where the plan's geometry rules that out
+    int win_ref;
     // fan: per (view, cell) the view-only quotients of the line chord,
     // (1/max(|a|,|b|), 1/(|a||b|)), at [iv*nd + id]
     const float2* line_q;
