@@ -86,6 +86,7 @@ __global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y
     const float yc = (float)iy + 0.5f - (float)(g.ny / 2);
     const float xc = (float)ix + 0.5f - (float)(g.nx / 2);
     const float hz = g.dz / 2;
+    const ShadowMap ma = fl_shadow_map(g.ya0, g.inv_pa, 0.05f), mb = fl_shadow_map(g.zb0, g.inv_pb, 0.05f);
     float zc[KZ], acc[KZ];
 #pragma unroll
     for (int k = 0; k < KZ; k++) {
@@ -109,7 +110,7 @@ __global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y
             // shadow window (clamped to the panel), cut by the reference's
             // window where that can bind (the clamps of the latter are
             // implied by the former's)
-            const ShadowCols sc = fl_shadow_cols(c, s, g.SO, SD, xc, yc, g.ya0, g.inv_pa, 0.05f, na);
+            const ShadowCols sc = fl_shadow_cols(c, s, g.SO, SD, xc, yc, ma, na);
             alo = sc.alo;
             ahi = sc.ahi;
             if (g.win_ref & 1) {
@@ -125,7 +126,7 @@ __global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y
                 if (iz0 + k < g.nz && alo <= ahi) {
                     const float zr = zc[k] - sz;
                     int bl, bh;
-                    fl_shadow_rows(sc, zr, hz, g.zb0, g.inv_pb, 0.05f, nb, bl, bh);
+                    fl_shadow_rows(sc, zr, hz, mb, nb, bl, bh);
                     if (rows_ref) {
                         bh = min(bh, (int)ceilf((zr + d) * t + (float)nb2));
                         bl = max(bl, (int)floorf((zr - d) * t + (float)nb2));
@@ -178,8 +179,9 @@ __global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y
             }
             __syncthreads();
             // the ratios' IEEE quotients from the staged reciprocals
+            const uint32_t mA = fl_small_div_m((uint32_t)nA);   // k * nA < 2^18 in these tables
             for (int k = tid; k < nA * nB; k += 256) {
-                const int ka = k % nA, kb = k / nA;
+                const int kb = fl_small_div(k, mA), ka = k - kb * nA;
                 const float a = c_a[ka], bb = c_b[ka], cc = r_c[kb], rc = r_rc[kb];
                 q_r[k] = make_float4(fl_div_r(cc, a, c_ra[ka]), fl_div_r(cc, bb, c_rb[ka]), fl_div_r(a, cc, rc),
                                      fl_div_r(bb, cc, rc));

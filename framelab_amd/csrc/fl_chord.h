@@ -255,69 +255,51 @@ FL_HD float fl_chord_box3_fast(const float* p, const float* h, bool xmajor, cons
 }
 
 // Detector cells whose ray can meet a voxel box, from the projection of the
-// box corners (shadow bounding box) widened by `margin` cells.  Rotated
+// box corners (shadow bounding box) widened by a margin in cells.  Rotated
 // frame: xr along the source->centre axis, yr along the detector row; the
 // ray of column a passes the detector at u = y_det[a] and meets a point
 // (xr, yr) when u = yr*SD/(xr+SO) (likewise rows: v = z_det[b] and
 // zr*SD/(xr+SO), zr relative to the source height).  Corner offsets of the
 // rotated unit square are (c*dx+s*dy, -s*dx+c*dy), dx,dy = +-1/2.
-// Hardware reciprocals are enough here: the margin (a twentieth of a cell)
-// dwarfs their rounding.
-struct ShadowWin {
-    int alo, ahi, blo, bhi;
+// Hardware reciprocals and the one-FMA index maps are enough here: the
+// margin (a twentieth of a cell) dwarfs their rounding.
+
+// Coordinate -> widened cell index bounds of a uniform grid with first
+// coordinate p0 and pitch 1/inv_p: index = u*inv_p + (klo or khi).
+struct ShadowMap {
+    float inv_p, klo, khi;
 };
 
-// The two halves of fl_shadow_window, for a column of voxels that share
-// (xc, yc): the column range depends on the xy square only; the row range
-// also on the voxel's height zr (relative to the source) and half-height.
+__host__ __device__ inline ShadowMap fl_shadow_map(float p0, float inv_p, float margin)
+{
+    ShadowMap m;
+    m.inv_p = inv_p;
+    m.klo = -p0 * inv_p - margin;
+    m.khi = -p0 * inv_p + margin;
+    return m;
+}
+
+// Cells [lo, hi] (clamped to [0, n-1]) whose coordinates the closed range
+// [umin, umax], widened by the margin, reaches; a negative pitch swaps ends.
+__device__ __forceinline__ void fl_shadow_range(const ShadowMap& m, float umin, float umax, int n, int& lo, int& hi)
+{
+    const bool up = m.inv_p > 0;
+    lo = (int)ceilf(fmaf(up ? umin : umax, m.inv_p, m.klo));
+    hi = (int)floorf(fmaf(up ? umax : umin, m.inv_p, m.khi));
+    if (lo < 0) lo = 0;
+    if (hi > n - 1) hi = n - 1;
+}
+
+// Column range of a column of voxels that share (xc, yc), and what their
+// row ranges need: the rows depend also on each voxel's height zr
+// (relative to the source) and half-height.
 struct ShadowCols {
     int alo, ahi;
     float rn, rf;   // SD / depth of the near and far ends of the square
 };
 
 __device__ __forceinline__ ShadowCols fl_shadow_cols(float c, float s, float SO, float SD, float xc, float yc,
-                                                     float ya0, float inv_pa, float margin, int na)
-{
-    const float xr = c * xc + s * yc, yr = -s * xc + c * yc;
-    const float hc = 0.5f * c, hs = 0.5f * s;
-    float umin = FL_INF, umax = -FL_INF;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const float ox = (k & 1) ? hc : -hc, oy = (k & 2) ? hs : -hs;
-        const float xk = xr + ox + oy;
-        const float yk = yr + ((k & 1) ? -hs : hs) + ((k & 2) ? hc : -hc);
-        const float u = yk * SD * __builtin_amdgcn_rcpf(xk + SO);
-        umin = fminf(umin, u);
-        umax = fmaxf(umax, u);
-    }
-    const float ex = 0.5f * (fabsf(c) + fabsf(s));
-    ShadowCols w;
-    w.rn = SD * __builtin_amdgcn_rcpf(xr - ex + SO);
-    w.rf = SD * __builtin_amdgcn_rcpf(xr + ex + SO);
-    const float a1 = (umin - ya0) * inv_pa, a2 = (umax - ya0) * inv_pa;
-    w.alo = (int)ceilf(fminf(a1, a2) - margin);
-    w.ahi = (int)floorf(fmaxf(a1, a2) + margin);
-    if (w.alo < 0) w.alo = 0;
-    if (w.ahi > na - 1) w.ahi = na - 1;
-    return w;
-}
-
-__device__ __forceinline__ void fl_shadow_rows(const ShadowCols& sc, float zr, float hz, float zb0, float inv_pb,
-                                               float margin, int nb, int& blo, int& bhi)
-{
-    const float zl = zr - hz, zh = zr + hz;
-    const float vmax = zh > 0 ? zh * sc.rn : zh * sc.rf;
-    const float vmin = zl < 0 ? zl * sc.rn : zl * sc.rf;
-    const float b1 = (vmin - zb0) * inv_pb, b2 = (vmax - zb0) * inv_pb;
-    blo = (int)ceilf(fminf(b1, b2) - margin);
-    bhi = (int)floorf(fmaxf(b1, b2) + margin);
-    if (blo < 0) blo = 0;
-    if (bhi > nb - 1) bhi = nb - 1;
-}
-
-__device__ __forceinline__ ShadowWin fl_shadow_window(float c, float s, float SO, float SD, float xc, float yc,
-                                                     float zr, float hz, float ya0, float inv_pa, float zb0,
-                                                     float inv_pb, float margin, int na, int nb)
+                                                     const ShadowMap& ma, int na)
 {
     const float xr = c * xc + s * yc, yr = -s * xc + c * yc;
     const float hc = 0.5f * c, hs = 0.5f * s;
@@ -332,21 +314,22 @@ __device__ __forceinline__ ShadowWin fl_shadow_window(float c, float s, float SO
         umax = fmaxf(umax, u);
     }
     const float ex = 0.5f * (fabsf(c) + fabsf(s));
-    const float rn = SD * __builtin_amdgcn_rcpf(xr - ex + SO), rf = SD * __builtin_amdgcn_rcpf(xr + ex + SO);
-    const float zl = zr - hz, zh = zr + hz;
-    const float vmax = zh > 0 ? zh * rn : zh * rf;
-    const float vmin = zl < 0 ? zl * rn : zl * rf;
-    ShadowWin w;
-    // (a pitch may be negative: order the mapped ends)
-    const float a1 = (umin - ya0) * inv_pa, a2 = (umax - ya0) * inv_pa;
-    const float b1 = (vmin - zb0) * inv_pb, b2 = (vmax - zb0) * inv_pb;
-    w.alo = (int)ceilf(fminf(a1, a2) - margin);
-    w.ahi = (int)floorf(fmaxf(a1, a2) + margin);
-    w.blo = (int)ceilf(fminf(b1, b2) - margin);
-    w.bhi = (int)floorf(fmaxf(b1, b2) + margin);
-    if (w.alo < 0) w.alo = 0;
-    if (w.ahi > na - 1) w.ahi = na - 1;
-    if (w.blo < 0) w.blo = 0;
-    if (w.bhi > nb - 1) w.bhi = nb - 1;
+    ShadowCols w;
+    w.rn = SD * __builtin_amdgcn_rcpf(xr - ex + SO);
+    w.rf = SD * __builtin_amdgcn_rcpf(xr + ex + SO);
+    fl_shadow_range(ma, umin, umax, na, w.alo, w.ahi);
     return w;
+}
+
+// Row range of the voxel at height zr (half-height hz) in that column: the
+// top edge zr+hz projects highest through the square's near end if it lies
+// above the source height and through its far end otherwise; the bottom
+// edge zr-hz lowest through the near end if below, the far end otherwise.
+__device__ __forceinline__ void fl_shadow_rows(const ShadowCols& sc, float zr, float hz, const ShadowMap& mb, int nb,
+                                               int& blo, int& bhi)
+{
+    const float zl = zr - hz, zh = zr + hz;
+    const float vmax = zh * (zh > 0 ? sc.rn : sc.rf);
+    const float vmin = zl * (zl < 0 ? sc.rn : sc.rf);
+    fl_shadow_range(mb, vmin, vmax, nb, blo, bhi);
 }

@@ -39,6 +39,12 @@ __device__ __forceinline__ float fl_div_r(float num, float d, float r)
     return __builtin_isinf(r) ? q : v;
 }
 
+// k / n for 0 <= k, 1 <= n with k * n < 2^18, by one multiply and shift:
+// with m = ceil(2^18 / n) = (2^18 + e) / n, e < n, k*m / 2^18 exceeds k / n
+// by k*e / (n 2^18) < 1/n, which never reaches the next integer.
+__device__ __forceinline__ uint32_t fl_small_div_m(uint32_t n) { return ((1u << 18) + n - 1) / n; }
+__device__ __forceinline__ int fl_small_div(int k, uint32_t m) { return (int)(((uint32_t)k * m) >> 18); }
+
 __device__ __forceinline__ float fl_load(const float* p, size_t i) { return p[i]; }
 __device__ __forceinline__ float fl_load(const bf16_t* p, size_t i)
 {

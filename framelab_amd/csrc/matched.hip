@@ -86,6 +86,7 @@ __global__ __launch_bounds__(256, 6) void k_adj_matched_cone(const YT* __restric
     const float xc = (float)ix + 0.5f - (float)(f.nx / 2);
     const float yc = (float)iy + 0.5f - (float)(f.ny / 2);
     const float hz = 0.5f * f.dz0;
+    const ShadowMap ma = fl_shadow_map(f.ya0, f.inv_pa, 0.05f), mb = fl_shadow_map(f.zb0, f.inv_pb, 0.05f);
     const bool dz_one = f.dz0 == 1.0f;
     float acc[KZ];
 #pragma unroll
@@ -101,7 +102,7 @@ __global__ __launch_bounds__(256, 6) void k_adj_matched_cone(const YT* __restric
 #pragma unroll
         for (int k = 0; k < KZ; k++) { blo[k] = nb; bhi[k] = -1; }
         if (col) {
-            const ShadowCols sc = fl_shadow_cols(c, s, f.SO, SD, xc, yc, f.ya0, f.inv_pa, 0.05f, na);
+            const ShadowCols sc = fl_shadow_cols(c, s, f.SO, SD, xc, yc, ma, na);
             alo = sc.alo;
             ahi = sc.ahi;
 #pragma unroll
@@ -110,7 +111,7 @@ __global__ __launch_bounds__(256, 6) void k_adj_matched_cone(const YT* __restric
                 if (iz < f.nz && alo <= ahi) {
                     const float zc = ((float)iz + 0.5f - (float)(f.nz / 2)) * f.dz0;
                     int bl, bh;
-                    fl_shadow_rows(sc, zc - sz, hz, f.zb0, f.inv_pb, 0.05f, nb, bl, bh);
+                    fl_shadow_rows(sc, zc - sz, hz, mb, nb, bl, bh);
                     if (bl <= bh) {
                         blo[k] = bl; bhi[k] = bh;
                         ublo = min(ublo, bl); ubhi = max(ubhi, bh);
@@ -161,8 +162,9 @@ __global__ __launch_bounds__(256, 6) void k_adj_matched_cone(const YT* __restric
                 const int kb = k / (BZ + 1), q = k % (BZ + 1);
                 t_az[kb][q] = fl_div_r((float)(iz0 + q - f.nz / 2) * f.dz0 - z1, t_dz[kb], t_rz[kb]);
             }
+            const uint32_t mA = fl_small_div_m((uint32_t)nA);   // k * nA < 2^18 in these tables
             for (int k = tid; k < nA * nB; k += 256) {
-                const int ka = k % nA, kb = k / nA;
+                const int kb = fl_small_div(k, mA), ka = k - kb * nA;
                 const float dx = t_dx[ka], dy = t_dy[ka], dz = t_dz[kb];
                 t_L[k] = sqrtf(dx * dx + dy * dy + dz * dz);
                 t_y[k] = fl_load(y, (size_t)(B0 + kb) * na + (A0 + ka));
