@@ -63,6 +63,7 @@ __global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y
 {
     __shared__ float c_a[FP_A], c_b[FP_A], c_sba[FP_A], c_sab[FP_A];
     __shared__ float c_ra[FP_A], c_rb[FP_A];
+    __shared__ float c_hx[FP_A], c_hy[FP_A];   // box half-sides signed by the ray's direction
     __shared__ float r_c[FP_B], r_rc[FP_B];
     __shared__ float4 q_r[FP_A * FP_B];
     __shared__ float q_y[FP_A * FP_B];
@@ -169,6 +170,8 @@ __global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y
                 const float ra = 1.0f / a, rb = 1.0f / bb;
                 c_a[k] = a; c_b[k] = bb;
                 c_ra[k] = ra; c_rb[k] = rb;
+                c_hx[k] = a > 0 ? 0.5f : -0.5f;
+                c_hy[k] = bb > 0 ? 0.5f : -0.5f;
                 if (a == 0 || bb == 0) s_special[j & 1] = 1;
                 c_sba[k] = fl_div_r(bb, a, ra); c_sab[k] = fl_div_r(a, bb, rb);
             }
@@ -197,12 +200,11 @@ __global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y
                     const float p[3] = {x1 - xc, y1 - yc, z1 - zc[k]};
                     for (int ib = blo[k]; ib <= bhi[k]; ib++) {
                         const int kb = ib - B0;
-                        const float cc = r_c[kb];
+                        const float hzs = r_c[kb] > 0 ? hz : -hz;
                         for (int ia = alo; ia <= ahi; ia++) {
                             const int ka = ia - A0;
-                            const float a = c_a[ka], bb = c_b[ka];
                             float l;
-                            if (decltype(special)::value && (a == 0 || bb == 0)) {
+                            if (decltype(special)::value && (c_a[ka] == 0 || c_b[ka] == 0)) {
                                 const float yd = g.ydet[ia];
                                 l = fl_chord_box3(x1, y1, z1, c * g.OD - s * yd, s * g.OD + c * yd,
                                                   g.zdet[ib] + sz, 1.0f, 1.0f, g.dz, xc, yc, zc[k]);
@@ -211,10 +213,8 @@ __global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y
                                 FlRatiosS q;
                                 q.r.sba = c_sba[ka]; q.r.sab = c_sab[ka];
                                 q.r.sca = r4.x; q.r.scb = r4.y; q.r.sac = r4.z; q.r.sbc = r4.w;
-                                q.sg[0] = a > 0 ? 1.0f : -1.0f;
-                                q.sg[1] = bb > 0 ? 1.0f : -1.0f;
-                                q.sg[2] = cc > 0 ? 1.0f : -1.0f;
-                                l = fl_chord_box3_fast(p, h, fl_absf(a) > fl_absf(bb), q);
+                                q.hs[0] = c_hx[ka]; q.hs[1] = c_hy[ka]; q.hs[2] = hzs;
+                                l = fl_chord_box3_fast(p, h, q, [&] { return fl_absf(c_a[ka]) > fl_absf(c_b[ka]); });
                             }
                             acc[k] = acc[k] + l * q_y[kb * nA + ka];
                         }

@@ -196,29 +196,52 @@ FL_HD float fl_chord_box3_r(const float* p, const float* h, bool xmajor, const F
     return 0.0f;
 }
 
+// sqrtf(s) for finite s >= 0: the hardware square root and the
+// round-to-nearest test of its two neighbours (the correctly rounded
+// expansion the compiler emits), without that expansion's rescaling of
+// inputs below 2^-96 -- those take sqrtf itself -- and without its
+// special-value fix-up, which the one special input left, +0, does not need
+// (the neighbour tests keep the hardware's +0).  Bit-identical to sqrtf.
+__device__ __forceinline__ float fl_sqrt_nonneg(float s)
+{
+    float r = __builtin_amdgcn_sqrtf(s);
+    const float rd = __uint_as_float(__float_as_uint(r) - 1u), ru = __uint_as_float(__float_as_uint(r) + 1u);
+    const float ed = fmaf(-rd, r, s), eu = fmaf(-ru, r, s);
+    r = ed <= 0.0f ? rd : r;
+    r = eu > 0.0f ? ru : r;
+    if (s < 0x1p-96f && s > 0.0f) {
+        asm volatile("");   // keeps the rare path a branch (not speculated for every lane)
+        r = sqrtf(s);
+    }
+    return r;
+}
+
 // Branch-free form of fl_chord_box3_r for the generic case.  For each face
 // pair the candidate on the face the ray enters through (the -h face when
 // the direction along that axis is positive) and the one it leaves through
 // are formed with the same operations as in fl_face_pair_r: with the sign
 // s of the direction, mu - s*wu equals the lo (s > 0) or hi (s < 0)
-// candidate bit for bit, since x - (-y) and x + y round the same sum.  When
-// exactly one entry and one exit candidate pass, they are the only passing
-// candidates, so they are the two points fl_chord_box3_r takes (in some
-// order; the squared differences do not depend on it) and the length is
-// bit-identical.  With fewer than two passing candidates the length is 0;
-// every other pattern (ray through an edge or corner) takes the ordered
-// routine.  sg = (sign a, sign b, sign c) as +-1.
+// candidate bit for bit, since x - (-y) and x + y round the same sum, and
+// s*wu = r*(s*h) rounds to (r*h)*s exactly (a sign flip, and rounding is
+// sign-symmetric).  When exactly one entry and one exit candidate pass,
+// they are the only passing candidates, so they are the two points
+// fl_chord_box3_r takes (in some order; the squared differences do not
+// depend on it) and the length is bit-identical.  With fewer than two
+// passing candidates the length is 0; every other pattern (ray through an
+// edge or corner) takes the ordered routine, with xmajor() = |a| > |b|.
+// hs = signed half sides (sign a * h[0], sign b * h[1], sign c * h[2]).
+// The candidate counts are wave-wide lane masks (scalar-unit logic).
 struct FlRatiosS {
     FlRatios r;
-    float sg[3];
+    float hs[3];
 };
 
-FL_HD float fl_chord_box3_fast(const float* p, const float* h, bool xmajor, const FlRatiosS& q)
+template <typename XMajor>
+__device__ __forceinline__ float fl_chord_box3_fast(const float* p, const float* h, const FlRatiosS& q,
+                                                    XMajor xmajor)
 {
     const FlRatios& r = q.r;
-    // signed half-sides: r*(h*sg) rounds to (r*h)*sg exactly, since the
-    // factor sg = +-1 only flips the sign and rounding is sign-symmetric
-    const float hx = h[0] * q.sg[0], hy = h[1] * q.sg[1], hz = h[2] * q.sg[2];
+    const float hx = q.hs[0], hy = q.hs[1], hz = q.hs[2];
     // x faces: U = y (slope sba), V = z (slope sca)
     const float mxu = r.sba * (-p[0]) + p[1], wxu = r.sba * hx;
     const float mxv = r.sca * (-p[0]) + p[2], wxv = r.sca * hx;
@@ -232,15 +255,22 @@ FL_HD float fl_chord_box3_fast(const float* p, const float* h, bool xmajor, cons
     const float ex_u = mxu - wxu, ex_v = mxv - wxv, xx_u = mxu + wxu, xx_v = mxv + wxv;
     const float ey_u = myu - wyu, ey_v = myv - wyv, xy_u = myu + wyu, xy_v = myv + wyv;
     const float ez_u = mzu - wzu, ez_v = mzv - wzv, xz_u = mzu + wzu, xz_v = mzv + wzv;
-    const bool fex = fl_absf(ex_u) <= h[1] && fl_absf(ex_v) <= h[2];
-    const bool fxx = fl_absf(xx_u) <= h[1] && fl_absf(xx_v) <= h[2];
-    const bool fey = fl_absf(ey_u) <= h[0] && fl_absf(ey_v) <= h[2];
-    const bool fxy = fl_absf(xy_u) <= h[0] && fl_absf(xy_v) <= h[2];
-    const bool fez = fl_absf(ez_u) <= h[0] && fl_absf(ez_v) <= h[1];
-    const bool fxz = fl_absf(xz_u) <= h[0] && fl_absf(xz_v) <= h[1];
-    const int ne = (int)fex + (int)fey + (int)fez, nx = (int)fxx + (int)fxy + (int)fxz;
-    // one entry and one exit candidate (the common case): the chord between
-    // them, formed without a branch (lanes of other cases discard it)
+    // which candidates lie on their faces, as lane masks of the wave (one
+    // compare each; the logic below runs on the scalar unit)
+    auto in = [](float c, float hh) { return __builtin_amdgcn_ballot_w64(fl_absf(c) <= hh); };
+    const uint64_t bex = in(ex_u, h[1]) & in(ex_v, h[2]), bxx = in(xx_u, h[1]) & in(xx_v, h[2]);
+    const uint64_t bey = in(ey_u, h[0]) & in(ey_v, h[2]), bxy = in(xy_u, h[0]) & in(xy_v, h[2]);
+    const uint64_t bez = in(ez_u, h[0]) & in(ez_v, h[1]), bxz = in(xz_u, h[0]) & in(xz_v, h[1]);
+    const bool fex = __builtin_amdgcn_inverse_ballot_w64(bex), fxx = __builtin_amdgcn_inverse_ballot_w64(bxx);
+    const bool fey = __builtin_amdgcn_inverse_ballot_w64(bey), fxy = __builtin_amdgcn_inverse_ballot_w64(bxy);
+    // exactly one entry and one exit candidate (one), or at least two of
+    // either kind (edge and corner rays)
+    const uint64_t two_e = (bex & bey) | (bez & (bex | bey)), two_x = (bxx & bxy) | (bxz & (bxx | bxy));
+    const uint64_t any_e = bex | bey | bez, any_x = bxx | bxy | bxz;
+    const bool one = __builtin_amdgcn_inverse_ballot_w64(any_e & any_x & ~(two_e | two_x));
+    const bool corner = __builtin_amdgcn_inverse_ballot_w64(two_e | two_x);
+    // one entry and one exit candidate: the chord between them, formed
+    // without a branch (lanes of other cases discard it)
     const float e0 = fex ? -hx : (fey ? ey_u : ez_u);
     const float e1 = fex ? ex_u : (fey ? -hy : ez_v);
     const float e2 = fex ? ex_v : (fey ? ey_v : -hz);
@@ -248,9 +278,8 @@ FL_HD float fl_chord_box3_fast(const float* p, const float* h, bool xmajor, cons
     const float x1 = fxx ? xx_u : (fxy ? hy : xz_v);
     const float x2 = fxx ? xx_v : (fxy ? xy_v : hz);
     const float u = x0 - e0, v = x1 - e1, w = x2 - e2;
-    const bool one = ne == 1 && nx == 1;
-    float l = one ? sqrtf(u * u + v * v + w * w) : 0.0f;
-    if (!one && ne + nx > 1) l = fl_chord_box3_r(p, h, xmajor, r);   // edges and corners
+    float l = one ? fl_sqrt_nonneg(u * u + v * v + w * w) : 0.0f;
+    if (corner) l = fl_chord_box3_r(p, h, xmajor(), r);
     return l;
 }
 
