@@ -56,7 +56,7 @@ __device__ __forceinline__ int wave_max(int v) { return __ockl_wfred_max_i32(v);
 // the order of accumulation are those of the reference kernel
 // (Atx_cone_mf_gpu_new.cu:58-86: views in order, rows outer, columns
 // inner), restricted to the cells whose chord can be non-zero.
-// Five workgroups per CU (the LDS tables take 26.3 KB each; the register
+// Five workgroups per CU (the LDS tables take 26.6 KB each; the register
 // budget that allows five is the measured optimum at 256^3).
 template <typename YT, typename XT, int KZ>
 __global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y, XT* __restrict__ X, AdjGeom g)
