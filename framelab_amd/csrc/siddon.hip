@@ -70,13 +70,20 @@ __device__ __forceinline__ RayGeo fan_ray(const FwdGeom& g, int iv, int id)
 template <int BX, int BY>
 struct Bricks {
     int nbx, nby;
-    size_t frame;   // elements per bricked frame
+    unsigned row, plane;   // elements per row of bricks / per z plane
+    size_t frame;          // elements per bricked frame
     __host__ __device__ Bricks(int nx, int ny, int nz)
-        : nbx((nx + BX - 1) / BX), nby((ny + BY - 1) / BY), frame((size_t)nbx * nby * BX * BY * nz) {}
+        : nbx((nx + BX - 1) / BX), nby((ny + BY - 1) / BY), row((unsigned)nbx * BX * BY),
+          plane(row * (unsigned)nby), frame((size_t)plane * nz) {}
+    // (tx, ty, tz) in the volume, so non-negative: unsigned shifts and masks,
+    // and full-rate 24-bit multiplies while a plane holds under 2^24 elements
     __device__ __forceinline__ unsigned at(int tx, int ty, int tz) const
     {
-        if (BX == 1 && BY == 1) return ((unsigned)tz * nby + ty) * nbx + tx;
-        return (((unsigned)tz * nby + ty / BY) * nbx + tx / BX) * (BX * BY) + (ty % BY) * BX + tx % BX;
+        const unsigned x = (unsigned)tx, y = (unsigned)ty, z = (unsigned)tz;
+        if (BX == 1 && BY == 1) return (z * nby + y) * nbx + x;
+        const unsigned in = (x / BX) * (BX * BY) + (y % BY) * BX + x % BX;
+        if (plane < (1u << 24)) return __umul24(z, plane) + __umul24(y / BY, row) + in;
+        return z * plane + (y / BY) * row + in;
     }
 };
 
