@@ -44,6 +44,7 @@ __device__ __forceinline__ float matched_coef(float inx, float outx, float iny, 
 namespace {
 constexpr int MF_A = 48;   // footprint columns in LDS
 constexpr int MF_B = 32;   // footprint rows
+static_assert(MF_A <= 64 && MF_B <= 64, "footprint columns / rows are staged by one wave each");
 
 // wave-wide min / max (DPP row shifts and lane reads, no LDS round trips);
 // called with every lane of the wave active
@@ -138,14 +139,17 @@ __global__ __launch_bounds__(256, 6) void k_adj_matched_cone(const YT* __restric
         if (nA <= 0 || nB <= 0) continue;
         const float x1 = c * (-f.SO), y1 = s * (-f.SO), z1 = sz;
         if (nA <= MF_A && nB <= MF_B) {
-            // columns: direction components and the brick's x/y plane parameters
-            for (int k = tid; k < nA; k += 256) {
+            // columns (wave 0): direction components; rows (wave 1) likewise;
+            // both fit one wave, and their load and divide latencies overlap
+            if (tid < nA) {
+                const int k = tid;
                 const float yd = f.ydet[A0 + k];
                 const float dx = (c * f.OD - s * yd) - x1, dy = (s * f.OD + c * yd) - y1;
                 t_dx[k] = dx; t_dy[k] = dy;
                 t_rx[k] = 1.0f / dx; t_ry[k] = 1.0f / dy;
             }
-            for (int k = tid; k < nB; k += 256) {
+            if (tid >= 64 && tid - 64 < nB) {
+                const int k = tid - 64;
                 const float dz = (f.zdet[B0 + k] + sz) - z1;
                 t_dz[k] = dz;
                 t_rz[k] = 1.0f / dz;
