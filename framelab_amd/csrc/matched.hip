@@ -61,7 +61,7 @@ __device__ __forceinline__ int mwave_max(int v) { return __ockl_wfred_max_i32(v)
 // voxel then visits only the cells of its own shadow.  Views of each frame
 // in order, columns outer; FP32 accumulation.
 // Six workgroups per CU (measured optimum of the register budget).
-template <typename YT, typename XT, int KZ>
+template <typename YT, typename XT, int KZ, bool DZ1>
 __global__ __launch_bounds__(256, 6) void k_adj_matched_cone(const YT* __restrict__ Y, XT* __restrict__ X,
                                                          FwdGeom f, AdjGeom g)
 {
@@ -88,7 +88,9 @@ __global__ __launch_bounds__(256, 6) void k_adj_matched_cone(const YT* __restric
     const float yc = (float)iy + 0.5f - (float)(f.ny / 2);
     const float hz = 0.5f * f.dz0;
     const ShadowMap ma = fl_shadow_map(f.ya0, f.inv_pa, 0.05f), mb = fl_shadow_map(f.zb0, f.inv_pb, 0.05f);
-    const bool dz_one = f.dz0 == 1.0f;
+    // DZ1: z planes one voxel apart, so the z cell index needs no division
+    // (a compile-time choice: a runtime flag lets the divide be speculated)
+    constexpr bool dz_one = DZ1;
     float acc[KZ];
 #pragma unroll
     for (int k = 0; k < KZ; k++) acc[k] = 0.0f;
@@ -259,8 +261,12 @@ static hipError_t launch_matched_t(const void* Y, void* X, const fl_plan_s* p, h
     if (p->cone) {
         constexpr int KZ = 4;
         uint32_t nblk = (uint32_t)((f.nx + 7) / 8) * ((f.ny + 7) / 8) * ((f.nz + 4 * KZ - 1) / (4 * KZ));
-        hipLaunchKernelGGL((k_adj_matched_cone<YT, XT, KZ>), dim3(nblk, p->nt), dim3(256), 0, st, (const YT*)Y,
-                           (XT*)X, f, p->ag);
+        if (f.dz0 == 1.0f)
+            hipLaunchKernelGGL((k_adj_matched_cone<YT, XT, KZ, true>), dim3(nblk, p->nt), dim3(256), 0, st,
+                               (const YT*)Y, (XT*)X, f, p->ag);
+        else
+            hipLaunchKernelGGL((k_adj_matched_cone<YT, XT, KZ, false>), dim3(nblk, p->nt), dim3(256), 0, st,
+                               (const YT*)Y, (XT*)X, f, p->ag);
     } else {
         dim3 grid((f.nx + 15) / 16, (f.ny + 15) / 16, p->nt);
         hipLaunchKernelGGL((k_adj_matched_fan<YT, XT>), grid, dim3(16, 16), 0, st, (const YT*)Y, (XT*)X, f,

@@ -380,21 +380,23 @@ def _trace_transpose(oracle, p, Y):
     return out * float(p.scale)
 
 
-@pytest.mark.parametrize("geom", ["staged", "all_wide", "mixed"])
+@pytest.mark.parametrize("geom", ["staged", "all_wide", "mixed", "dz"])
 def test_adjoints_staged_and_direct_paths(gpu, oracle, geom):
     """Detector pitches fine against the voxels, so the per-workgroup
     footprint of a view exceeds the staged tables for every brick
     ("all_wide": 16^3 against 256 x 192 cells) or for the bricks near the
     source only ("mixed": 32^3 against 160 x 100), exercising the direct path
     of both adjoints beside the table path ("staged": the usual pitch, 32^3
-    against 64 x 48, tables only).  Exact adjoint bit-exact against
+    against 64 x 48, tables only; "dz": the same with voxels 0.8 high,
+    32 x 32 x 24, so z cell indices take the division).  Exact adjoint bit-exact against
     the oracle; matched adjoint against the FP64 sum of the oracle's FP32
     traversal coefficients (the transpose it equals coefficient by
     coefficient; only the FP32 accumulation differs)."""
     from framelab_amd.projector import ADJ_MATCHED
     _say(f"case: adjoint footprint paths ({geom})")
     p = {"staged": lambda: small_cone(nv=8), "all_wide": lambda: small_cone(256, 192, 4, 16),
-         "mixed": lambda: small_cone(160, 100, 6, 32)}[geom]()
+         "mixed": lambda: small_cone(160, 100, 6, 32),
+         "dz": lambda: _cone_dz()}[geom]()
     rng = np.random.default_rng(21)
     Y = rng.random((p.nv, p.nb, p.na), dtype=np.float32)
     P = _proj(p)
@@ -432,6 +434,12 @@ def _cone_full_fan(na, nb, nv, N, SO, OD):
     from framelab_amd import geometry as G
     return G.cone_para(na, nb, nv, N=N, SO=SO, OD=OD, Ly=G.full_fan_width(SO, SO + OD),
                        Lz=G.full_cone_height(SO, SO + OD))
+
+
+def _cone_dz():
+    from framelab_amd import geometry as G
+    return G.cone_para(64, 48, 8, N=(32, 32, 24), L=(2.0, 2.0, 1.2), SO=4.0, OD=4.0,
+                       Ly=G.full_fan_width(), Lz=G.full_cone_height())
 
 
 def test_axis_aligned_rays(gpu, oracle):
