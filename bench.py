@@ -249,6 +249,40 @@ def bench_batch(dev, cgls_iters=30, outer=5, inner=3):
     }
 
 
+def framelet_leg(x, nvox, reps=3):
+    """Event-timed 3-D framelet analysis (W u) and synthesis (W^T alpha) of
+    the volume x, with achieved bandwidth against the algorithmic bytes."""
+    import torch
+    from framelab_amd.framelets import FrameletSystem
+    W = FrameletSystem(dim=3, type="linear", level=1)
+    out = {"frame": "linear B-spline, 3-D, 27 bands, 1 level", "volume": list(x.shape)}
+    for cdt, name, sb in ((torch.float32, "f32", 4), (torch.bfloat16, "bf16", 2)):
+        a = W.forwardTransform(x, coef_dtype=cdt)
+        u = W.adjointTransform(a)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        ev[0].record()
+        for _ in range(reps):
+            W.forwardTransform(x, coef_dtype=cdt)
+        ev[1].record()
+        for _ in range(reps):
+            u = W.adjointTransform(a)
+        ev[2].record()
+        torch.cuda.synchronize()
+        rel = float((u - x).norm() / x.norm())   # W^T W = I
+        nbytes = nvox * (4 + W.nbands * sb)
+        for leg, t in (("analysis", ev[0].elapsed_time(ev[1]) / reps),
+                       ("synthesis", ev[1].elapsed_time(ev[2]) / reps)):
+            gbs = nbytes / (t * 1e-3) / 1e9
+            out[f"{name}_{leg}_ms"] = t
+            out[f"{name}_{leg}_gbs"] = gbs
+            out[f"{name}_{leg}_frac_hbm"] = gbs / HBM_PEAK_GBS
+        out[f"{name}_perfect_reconstruction_rel"] = rel
+        del a
+    log(f"bench: framelet analysis {out['f32_analysis_gbs']:.0f} GB/s, synthesis "
+        f"{out['f32_synthesis_gbs']:.0f} GB/s (FP32 coefficients)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -260,7 +294,8 @@ def main():
     ap.add_argument("--cpu-views", type=int, default=3)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 2D leg")
     ap.add_argument("--profile", action="store_true",
-                    help="counter run: one forward, one exact and one matched adjoint at 256^3, then a "
+                    help="counter run: one forward, one exact and one matched adjoint at 256^3, one "
+                         "framelet analysis and synthesis (27 bands, FP32), then a "
                          "plain 1 GiB device copy (the calibration kernel for HBM byte counters) and the "
                          "FMA-chain kernel (the calibration for VALU instruction counters); no JSON")
     args = ap.parse_args()
@@ -302,6 +337,9 @@ def main():
         P.forward(x, out=y)
         P.adjoint(y, out=xb, kind=ADJ_EXACT)
         P.adjoint(y, out=xb, kind=ADJ_MATCHED)
+        from framelab_amd.framelets import FrameletSystem
+        W3 = FrameletSystem(dim=3, type="linear", level=1)
+        W3.adjointTransform(W3.forwardTransform(x))
         src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
         dst.copy_(src)
@@ -313,7 +351,8 @@ def main():
             _lib.check(_lib.lib().fl_calibrate_valu(C.c_void_p(out.data_ptr()), out.numel(), 5000, kind,
                                                     C.c_void_p(torch.cuda.current_stream().cuda_stream)))
         torch.cuda.synchronize()
-        log("bench: profile run done (forward, exact adjoint, matched adjoint, 1 GiB copy, VALU calibration)")
+        log("bench: profile run done (forward, exact adjoint, matched adjoint, framelet analysis and "
+            "synthesis, 1 GiB copy, VALU calibration)")
         return
 
     log("bench: warmup")
@@ -365,6 +404,12 @@ def main():
         extra[f"{name}_t_At_matched_ms"] = b
         extra[f"{name}_matched_gups"] = 2.0 * nvox * nv / ((a + b) * 1e-3) / 1e9
 
+    # framelet analysis / synthesis on the same volume (3-D linear B-spline
+    # frame: 27 bands, one level), FP32 and BF16 coefficient storage.
+    # Algorithmic bytes per voxel: s_x + 27 s_b (read the volume, write every
+    # band; synthesis the reverse).
+    framelet = framelet_leg(x, nvox)
+
     result = {
         "metric": METRIC, "value": gups, "unit": "GUPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": mA + mAt, "higher_is_better": True, "scaling": "weak",
@@ -375,6 +420,7 @@ def main():
                    "phantom": "20 seeded ellipsoids (seed 2)"},
         "roofline": roof,
         "extra": extra,
+        "framelet": framelet,
     }
 
     if not args.no_cgls:

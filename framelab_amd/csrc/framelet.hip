@@ -17,7 +17,9 @@
 // One thread per voxel; analysis gathers the NT^d neighbourhood once and
 // produces all NF^d bands from it (separably, in registers), so the input is
 // read once and every band is written once (the transform is write-bound).
-// Synthesis gathers each band's neighbourhood and accumulates in FP32.
+// 3D synthesis runs separably on LDS tiles (k_iwt3s); 2D synthesis, and 3D
+// levels whose halo outgrows the tiles, gather each band's neighbourhood
+// per output.  All accumulate in FP32.
 #include "fl_common.h"
 #include "fl_framelet_dev.h"
 
@@ -155,6 +157,101 @@ __global__ __launch_bounds__(256) void k_iwt3(const CT* __restrict__ C, XT* u, i
     float v = alpha * acc;
     if (base) v = __builtin_fmaf(beta, base[idx], v);
     fl_store(u, idx, v);
+}
+
+// Separable 3D synthesis on LDS tiles.  The band sum factorises,
+//   u = sum_i corr_y[R_i]( sum_j corr_x[R_j]( sum_k corr_z[R_k] C_ijk ) ),
+// so a workgroup owning an IW_TX x IW_TY tile of (x, y) columns for `zc`
+// consecutive slices forms, per slice and per y filter i: the z pass of the
+// bands (i, j, *) for every j on the tile and its halo (in LDS), their x
+// pass on the tile's and halo's rows (in LDS), and adds the y pass to its
+// outputs.  Each coefficient is read NT times per halo position (its z
+// taps, mostly from L2) instead of NT^3 times.  FP32 FMAs in another
+// association than k_iwt3's: equal to it up to rounding.  Same in-place
+// contract (`base` may alias u).
+constexpr int IW_TX = 64, IW_TY = 16;
+constexpr int IW_OUT = IW_TX * IW_TY / 256;   // outputs per thread
+
+__host__ __device__ inline int iw_halo_positions(int h) { return (IW_TX + 2 * h) * (IW_TY + 2 * h); }
+// LDS: z pass [NF][PH][PW], x pass [PH][IW_TX], halo offsets [PH][PW] (int)
+template <int NF>
+__host__ __device__ inline size_t iw_lds_bytes(int h)
+{
+    return ((size_t)(NF + 1) * iw_halo_positions(h) + (size_t)(IW_TY + 2 * h) * IW_TX) * sizeof(float);
+}
+
+template <int NF, int NT, typename CT, typename XT>
+__global__ __launch_bounds__(256) void k_iwt3s(const CT* __restrict__ C, XT* u, int nx, int ny, int nz, int step,
+                                              int zc, FlTaps T, float alpha, float beta, const float* base)
+{
+    extern __shared__ float iw_lds[];
+    constexpr int CC = (NT - 1) / 2;
+    const int h = CC * step;
+    const int PW = IW_TX + 2 * h, PH = IW_TY + 2 * h, P = PW * PH;
+    float* tz = iw_lds;                                    // z pass of bands (i, j, *), per j
+    float* sx = iw_lds + NF * P;                           // x pass
+    int* poff = reinterpret_cast<int*>(sx + PH * IW_TX);   // halo position -> wrapped offset in a slice
+    const size_t nxy = (size_t)nx * ny, nvox = nxy * nz;
+    const int tilesx = (nx + IW_TX - 1) / IW_TX;
+    const int x0 = (int)(blockIdx.x % tilesx) * IW_TX, y0 = (int)(blockIdx.x / tilesx) * IW_TY;
+    const int tid = threadIdx.x;
+    for (int p = tid; p < P; p += 256) poff[p] = wrapi(y0 - h + p / PW, ny) * nx + wrapi(x0 - h + p % PW, nx);
+    __syncthreads();
+    for (int z = blockIdx.y * zc; z < min(nz, (int)(blockIdx.y + 1) * zc); z++) {
+        size_t zoff[NT];
+#pragma unroll
+        for (int kz = 0; kz < NT; kz++) zoff[kz] = (size_t)wrapi(z + (kz - CC) * step, nz) * nxy;
+        float acc[IW_OUT];
+#pragma unroll
+        for (int r = 0; r < IW_OUT; r++) acc[r] = 0.0f;
+#pragma unroll 1
+        for (int i = 0; i < NF; i++) {
+            for (int p = tid; p < P; p += 256) {
+                const int off = poff[p];
+#pragma unroll
+                for (int j = 0; j < NF; j++) {
+                    float t = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < NF; k++) {
+                        const CT* __restrict__ cb = C + (size_t)(i + NF * j + NF * NF * k) * nvox + off;
+#pragma unroll
+                        for (int kz = 0; kz < NT; kz++) t = __builtin_fmaf(T.R[k][kz], fl_load(cb, zoff[kz]), t);
+                    }
+                    tz[j * P + p] = t;
+                }
+            }
+            __syncthreads();
+            for (int q = tid; q < PH * IW_TX; q += 256) {
+                const float* row = tz + (q / IW_TX) * PW + q % IW_TX + h;
+                float s = 0.0f;
+#pragma unroll
+                for (int j = 0; j < NF; j++)
+#pragma unroll
+                    for (int kx = 0; kx < NT; kx++) s = __builtin_fmaf(T.R[j][kx], row[j * P + (kx - CC) * step], s);
+                sx[q] = s;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < IW_OUT; r++) {
+                const int o = tid + 256 * r, ox = o % IW_TX, oy = o / IW_TX;
+                float s = 0.0f;
+#pragma unroll
+                for (int ky = 0; ky < NT; ky++)
+                    s = __builtin_fmaf(T.R[i][ky], sx[(oy + h + (ky - CC) * step) * IW_TX + ox], s);
+                acc[r] += s;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < IW_OUT; r++) {
+            const int o = tid + 256 * r, x = x0 + o % IW_TX, y = y0 + o / IW_TX;
+            if (x < nx && y < ny) {
+                const size_t idx = (size_t)z * nxy + (size_t)y * nx + x;
+                float v = alpha * acc[r];
+                if (base) v = __builtin_fmaf(beta, base[idx], v);
+                fl_store(u, idx, v);
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------- 2D ------
@@ -407,9 +504,24 @@ static hipError_t iwt_t(const FlFrameDesc& f, const void* C, void* u, int level,
     const int step = 1 << (level - 1);
     const size_t n = (size_t)f.nx * f.ny * (f.dim == 3 ? f.nz : 1);
     if (f.dim == 3)
-        FL_FRAME_SWITCH(f.kind, hipLaunchKernelGGL((k_iwt3<NF, NT, CT, XT>), dim3(nblk(n)), dim3(256), 0, st,
-                                                   (const CT*)C, (XT*)u, f.nx, f.ny, f.nz, step, T, alpha,
-                                                   beta, base));
+        FL_FRAME_SWITCH(f.kind, {
+            // separable tiles while the halo tile fits 48 KB of LDS; the
+            // per-voxel gather beyond
+            const int h = ((NT - 1) / 2) * step;
+            const size_t lds = iw_lds_bytes<NF>(h);
+            if (lds <= 48 * 1024) {
+                const unsigned tiles = (unsigned)(((f.nx + IW_TX - 1) / IW_TX) * ((f.ny + IW_TY - 1) / IW_TY));
+                // a few slices per workgroup (their z taps overlap in L2),
+                // while that leaves >= 2048 workgroups
+                int zc = 1;
+                while (zc < 8 && (size_t)tiles * ((f.nz + 2 * zc - 1) / (2 * zc)) >= 2048) zc *= 2;
+                hipLaunchKernelGGL((k_iwt3s<NF, NT, CT, XT>), dim3(tiles, (f.nz + zc - 1) / zc), dim3(256), lds, st,
+                                   (const CT*)C, (XT*)u, f.nx, f.ny, f.nz, step, zc, T, alpha, beta, base);
+            } else {
+                hipLaunchKernelGGL((k_iwt3<NF, NT, CT, XT>), dim3(nblk(n)), dim3(256), 0, st, (const CT*)C, (XT*)u,
+                                   f.nx, f.ny, f.nz, step, T, alpha, beta, base);
+            }
+        });
     else
         FL_FRAME_SWITCH(f.kind, hipLaunchKernelGGL((k_iwt2<NF, NT, CT, XT>), dim3(nblk(n)), dim3(256), 0, st,
                                                    (const CT*)C, (XT*)u, f.nx, f.ny, step, T, alpha, beta,
