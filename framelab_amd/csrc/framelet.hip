@@ -254,6 +254,113 @@ __global__ __launch_bounds__(256) void k_iwt3s(const CT* __restrict__ C, XT* u, 
     }
 }
 
+// The same separable synthesis with the z reuse kept in registers, for
+// dilations 1 and 2 (compile-time STEP, so the z window is indexed
+// statically).  A workgroup owns the 64x16 tile for IZ_ZC output slices and
+// walks the band pairs (i, j): each thread loads, for its halo positions,
+// the NF bands (i, j, *) on all IZ_ZC + 2H input slices once and forms the
+// z pass of every output slice from registers (LDS, per slice); the x pass
+// is accumulated over j per slice (LDS), and after the last j the y pass
+// goes into the outputs.  Each coefficient is read (IZ_ZC + 2H) / IZ_ZC
+// times instead of NT times, without depending on L2 to keep the slices
+// between the passes that share them.
+constexpr int IZ_ZC = 4;   // output slices per workgroup
+
+template <int NT, int STEP>
+struct IzGeom {
+    static constexpr int CC = (NT - 1) / 2, H = CC * STEP;
+    static constexpr int PW = IW_TX + 2 * H, PH = IW_TY + 2 * H, P = PW * PH, WZ = IZ_ZC + 2 * H;
+    static constexpr size_t lds = ((size_t)IZ_ZC * P + (size_t)IZ_ZC * PH * IW_TX + P) * 4;
+};
+
+template <int NF, int NT, int STEP, typename CT, typename XT>
+__global__ __launch_bounds__(256) void k_iwt3z(const CT* __restrict__ C, XT* u, int nx, int ny, int nz, FlTaps T,
+                                              float alpha, float beta, const float* base)
+{
+    using G = IzGeom<NT, STEP>;
+    constexpr int CC = G::CC, H = G::H, PW = G::PW, PH = G::PH, P = G::P, WZ = G::WZ;
+    constexpr int SX = PH * IW_TX;
+    __shared__ float tzs[IZ_ZC][P];    // z pass of the bands (i, j, *), per output slice
+    __shared__ float sxs[IZ_ZC][SX];   // x pass, summed over j, per output slice
+    __shared__ int poff[P];            // halo position -> wrapped offset in a slice
+    const size_t nxy = (size_t)nx * ny, nvox = nxy * nz;
+    const int tilesx = (nx + IW_TX - 1) / IW_TX;
+    const int x0 = (int)(blockIdx.x % tilesx) * IW_TX, y0 = (int)(blockIdx.x / tilesx) * IW_TY;
+    const int z0 = (int)blockIdx.y * IZ_ZC;
+    const int tid = threadIdx.x;
+    for (int p = tid; p < P; p += 256) poff[p] = wrapi(y0 - H + p / PW, ny) * nx + wrapi(x0 - H + p % PW, nx);
+    size_t zoff[WZ];   // input slices z0 - H .. z0 + IZ_ZC - 1 + H (wrapped)
+#pragma unroll
+    for (int w = 0; w < WZ; w++) zoff[w] = (size_t)wrapi(z0 - H + w, nz) * nxy;
+    __syncthreads();
+    float acc[IZ_ZC][IW_OUT];
+#pragma unroll
+    for (int zz = 0; zz < IZ_ZC; zz++)
+#pragma unroll
+        for (int r = 0; r < IW_OUT; r++) acc[zz][r] = 0.0f;
+#pragma unroll 1
+    for (int i = 0; i < NF; i++) {
+#pragma unroll 1
+        for (int j = 0; j < NF; j++) {
+            for (int p = tid; p < P; p += 256) {
+                const int off = poff[p];
+                float v[NF][WZ];
+#pragma unroll
+                for (int k = 0; k < NF; k++) {
+                    const CT* __restrict__ cb = C + (size_t)(i + NF * j + NF * NF * k) * nvox + off;
+#pragma unroll
+                    for (int w = 0; w < WZ; w++) v[k][w] = fl_load(cb, zoff[w]);
+                }
+#pragma unroll
+                for (int zz = 0; zz < IZ_ZC; zz++) {
+                    float t = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < NF; k++)
+#pragma unroll
+                        for (int kz = 0; kz < NT; kz++) t = __builtin_fmaf(T.R[k][kz], v[k][zz + H + (kz - CC) * STEP], t);
+                    tzs[zz][p] = t;
+                }
+            }
+            __syncthreads();
+            for (int q = tid; q < IZ_ZC * SX; q += 256) {
+                const int zz = q / SX, q2 = q % SX;
+                const float* row = &tzs[zz][(q2 / IW_TX) * PW + q2 % IW_TX + H];
+                float s = 0.0f;
+#pragma unroll
+                for (int kx = 0; kx < NT; kx++) s = __builtin_fmaf(T.R[j][kx], row[(kx - CC) * STEP], s);
+                sxs[zz][q2] = j == 0 ? s : sxs[zz][q2] + s;
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int zz = 0; zz < IZ_ZC; zz++)
+#pragma unroll
+            for (int r = 0; r < IW_OUT; r++) {
+                const int o = tid + 256 * r, ox = o % IW_TX, oy = o / IW_TX;
+                float s = 0.0f;
+#pragma unroll
+                for (int ky = 0; ky < NT; ky++)
+                    s = __builtin_fmaf(T.R[i][ky], sxs[zz][(oy + H + (ky - CC) * STEP) * IW_TX + ox], s);
+                acc[zz][r] += s;
+            }
+    }
+#pragma unroll
+    for (int zz = 0; zz < IZ_ZC; zz++) {
+        const int z = z0 + zz;
+        if (z >= nz) break;
+#pragma unroll
+        for (int r = 0; r < IW_OUT; r++) {
+            const int o = tid + 256 * r, x = x0 + o % IW_TX, y = y0 + o / IW_TX;
+            if (x < nx && y < ny) {
+                const size_t idx = (size_t)z * nxy + (size_t)y * nx + x;
+                float v = alpha * acc[zz][r];
+                if (base) v = __builtin_fmaf(beta, base[idx], v);
+                fl_store(u, idx, v);
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- 2D ------
 
 template <int NF, int NT, typename XT, typename CT>
@@ -505,8 +612,23 @@ static hipError_t iwt_t(const FlFrameDesc& f, const void* C, void* u, int level,
     const size_t n = (size_t)f.nx * f.ny * (f.dim == 3 ? f.nz : 1);
     if (f.dim == 3)
         FL_FRAME_SWITCH(f.kind, {
-            // separable tiles while the halo tile fits 48 KB of LDS; the
-            // per-voxel gather beyond
+            // dilations 1 and 2: the z-window tiles (while their LDS fits
+            // 48 KB); then the separable tiles while the halo tile fits;
+            // the per-voxel gather beyond
+            const dim3 zgrid((unsigned)(((f.nx + IW_TX - 1) / IW_TX) * ((f.ny + IW_TY - 1) / IW_TY)),
+                             (unsigned)((f.nz + IZ_ZC - 1) / IZ_ZC));
+            if (step == 1 && IzGeom<NT, 1>::lds <= 48 * 1024) {
+                hipLaunchKernelGGL((k_iwt3z<NF, NT, 1, CT, XT>), zgrid, dim3(256), 0, st, (const CT*)C, (XT*)u, f.nx,
+                                   f.ny, f.nz, T, alpha, beta, base);
+                return hipGetLastError();
+            }
+            if constexpr (IzGeom<NT, 2>::lds <= 48 * 1024) {
+                if (step == 2) {
+                    hipLaunchKernelGGL((k_iwt3z<NF, NT, 2, CT, XT>), zgrid, dim3(256), 0, st, (const CT*)C, (XT*)u,
+                                       f.nx, f.ny, f.nz, T, alpha, beta, base);
+                    return hipGetLastError();
+                }
+            }
             const int h = ((NT - 1) / 2) * step;
             const size_t lds = iw_lds_bytes<NF>(h);
             if (lds <= 48 * 1024) {
