@@ -17,9 +17,10 @@
 // One thread per voxel; analysis gathers the NT^d neighbourhood once and
 // produces all NF^d bands from it (separably, in registers), so the input is
 // read once and every band is written once (the transform is write-bound).
-// 3D synthesis runs separably on LDS tiles (k_iwt3s); 2D synthesis, and 3D
-// levels whose halo outgrows the tiles, gather each band's neighbourhood
-// per output.  All accumulate in FP32.
+// 3D synthesis runs separably on LDS tiles (k_iwt3z for dilations 1 and 2,
+// k_iwt3s beyond); 2D synthesis, and 3D levels whose halo outgrows the
+// tiles, gather each band's neighbourhood per output.  All accumulate in
+// FP32.
 #include "fl_common.h"
 #include "fl_framelet_dev.h"
 
