@@ -29,16 +29,17 @@
 // Per (ray, voxel) pair of one view: the Siddon segment of the ray inside
 // the voxel, credited when the forward's midpoint rule maps it there.
 // in/out: entry/exit plane parameters per axis.  Returns the length
-// coefficient (a_out - a_in)*|d|, or 0.
+// coefficient (a_out - a_in)*|d|, or 0.  Branch-free: the three cell tests
+// cost less than the divergent branches around them.
 __device__ __forceinline__ float matched_coef(float inx, float outx, float iny, float outy, float inz, float outz,
                                               const SidAxis& ax, const SidAxis& ay, const SidAxis& az, int ix,
                                               int iy, int iz, bool dz_one, float L)
 {
     const float ain = fmaxf(fmaxf(inx, iny), inz), aout = fminf(fminf(outx, outy), outz);
-    if (!(aout > ain)) return 0.0f;
     const float m = (ain + aout) * 0.5f;
-    if (sid_cell_xy(ax, m) != ix || sid_cell_xy(ay, m) != iy || sid_cell_z(az, m, dz_one) != iz) return 0.0f;
-    return (aout - ain) * L;
+    const bool hit = (aout > ain) & (sid_cell_xy(ax, m) == ix) & (sid_cell_xy(ay, m) == iy) &
+                     (sid_cell_z(az, m, dz_one) == iz);
+    return hit ? (aout - ain) * L : 0.0f;
 }
 
 namespace {
@@ -205,7 +206,7 @@ __global__ __launch_bounds__(256, 6) void k_adj_matched_cone(const YT* __restric
                         const int q = kb * nA + ka;
                         const float Si = matched_coef(inx, outx, iny, outy, inz, outz, ax, ay, az, ix, iy, iz,
                                                       dz_one, t_L[q]);
-                        if (Si != 0.0f) acc[k] = acc[k] + Si * t_y[q];
+                        acc[k] = acc[k] + Si * t_y[q];   // (+-0 for a miss: the sum is unchanged)
                     }
                 }
             }
