@@ -37,8 +37,9 @@ __device__ __forceinline__ float matched_coef(float inx, float outx, float iny, 
 {
     const float ain = fmaxf(fmaxf(inx, iny), inz), aout = fminf(fminf(outx, outy), outz);
     const float m = (ain + aout) * 0.5f;
-    const bool hit = (aout > ain) & (sid_cell_xy(ax, m) == ix) & (sid_cell_xy(ay, m) == iy) &
-                     (sid_cell_z(az, m, dz_one) == iz);
+    // (every test evaluated: no short-circuit branches)
+    const int hit = int(aout > ain) & int(sid_cell_xy(ax, m) == ix) & int(sid_cell_xy(ay, m) == iy) &
+                    int(sid_cell_z(az, m, dz_one) == iz);
     return hit ? (aout - ain) * L : 0.0f;
 }
 
