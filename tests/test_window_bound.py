@@ -8,12 +8,7 @@ several volume heights, and panels shifted against the windows' grid.
 Weakening any term (the rotated square's extent, the nearest depth, the
 helix's z range, the grid-offset tolerance) lets a cut through.  The bound
 is restated here from the plan's (csrc/capi.hip: cone_ref_window_bits);
-the GPU parity tests cover the kernel on both sides of it."""e inside the
-reference's window, for every view.  Checked by brute force over random
-geometries (sizes, source distance, pitches, voxel height, helical source
-offsets, panels shifted against the windows' grid), in FP64.  The bound is restated here from the plan's
-(csrc/capi.hip: cone_ref_window_bits); the GPU parity tests cover the
-kernel on both sides of it."""
+the GPU parity tests cover the kernel on both sides of it."""
 import math
 
 import numpy as np
