@@ -21,12 +21,19 @@
 // k_iwt3s beyond); 2D synthesis, and 3D levels whose halo outgrows the
 // tiles, gather each band's neighbourhood per output.  All accumulate in
 // FP32.
+#include <algorithm>
+
 #include "fl_common.h"
 #include "fl_framelet_dev.h"
 
 // ---------------------------------------------------------------- 3D ------
 
-template <int NF, int NT, typename XT>
+// Coordinate v in [-n, 2n) wrapped into [0, n) without a divide.
+__device__ __forceinline__ int wrap_near(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+// NEAR: the taps reach at most one period beyond the volume (C*step <= n
+// on every axis), so wrap_near serves.
+template <int NF, int NT, bool NEAR = false, typename XT>
 __device__ __forceinline__ void gather3(const XT* __restrict__ u, int nx, int ny, int nz, int x, int y,
                                         int z, int step, float (&nb)[NT][NT][NT])
 {
@@ -34,9 +41,9 @@ __device__ __forceinline__ void gather3(const XT* __restrict__ u, int nx, int ny
     int xs[NT], ys[NT], zs[NT];
 #pragma unroll
     for (int k = 0; k < NT; k++) {
-        xs[k] = wrapi(x + (k - C) * step, nx);
-        ys[k] = wrapi(y + (k - C) * step, ny);
-        zs[k] = wrapi(z + (k - C) * step, nz);
+        xs[k] = NEAR ? wrap_near(x + (k - C) * step, nx) : wrapi(x + (k - C) * step, nx);
+        ys[k] = NEAR ? wrap_near(y + (k - C) * step, ny) : wrapi(y + (k - C) * step, ny);
+        zs[k] = NEAR ? wrap_near(z + (k - C) * step, nz) : wrapi(z + (k - C) * step, nz);
     }
 #pragma unroll
     for (int kz = 0; kz < NT; kz++)
@@ -89,16 +96,20 @@ __device__ __forceinline__ void bands3(const FlTaps& T, const float (&nb)[NT][NT
             }
 }
 
-template <int NF, int NT, typename XT, typename CT>
+// Workgroups of 64 x 4 (x, y) voxels of one slice: coordinates without
+// divides.
+template <int NF, int NT, bool NEAR, typename XT, typename CT>
 __global__ __launch_bounds__(256) void k_fwt3(const XT* __restrict__ u, CT* __restrict__ C, int nx, int ny,
-                                             int nz, int step, FlTaps T)
+                                             int nz, int z0, int step, FlTaps T)
 {
     const size_t nvox = (size_t)nx * ny * nz;
-    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (idx >= nvox) return;
-    const int x = (int)(idx % nx), y = (int)((idx / nx) % ny), z = (int)(idx / ((size_t)nx * ny));
+    const int x = (int)blockIdx.x * 64 + (int)(threadIdx.x & 63);
+    const int y = (int)blockIdx.y * 4 + (int)(threadIdx.x >> 6);
+    const int z = z0 + (int)blockIdx.z;
+    if (x >= nx || y >= ny) return;
+    const size_t idx = ((size_t)z * ny + y) * nx + x;
     float nb[NT][NT][NT];
-    gather3<NF, NT>(u, nx, ny, nz, x, y, z, step, nb);
+    gather3<NF, NT, NEAR>(u, nx, ny, nz, x, y, z, step, nb);
     float out[NF * NF * NF];
     bands3<NF, NT>(T, nb, out);
 #pragma unroll
@@ -596,8 +607,21 @@ static hipError_t fwt_t(const FlFrameDesc& f, const void* u, void* C, int level,
     const int step = 1 << (level - 1);
     const size_t n = (size_t)f.nx * f.ny * (f.dim == 3 ? f.nz : 1);
     if (f.dim == 3)
-        FL_FRAME_SWITCH(f.kind, hipLaunchKernelGGL((k_fwt3<NF, NT, XT, CT>), dim3(nblk(n)), dim3(256), 0, st,
-                                                   (const XT*)u, (CT*)C, f.nx, f.ny, f.nz, step, T));
+        FL_FRAME_SWITCH(f.kind, {
+            if ((f.ny + 3) / 4 > 65535) return hipErrorInvalidValue;
+            const bool near = (NT - 1) / 2 * step <= std::min(f.nx, std::min(f.ny, f.nz));
+            // slices in launches of at most 65535 (the grid's z limit)
+            for (int z0 = 0; z0 < f.nz; z0 += 65535) {
+                const dim3 grid((unsigned)((f.nx + 63) / 64), (unsigned)((f.ny + 3) / 4),
+                                (unsigned)std::min(65535, f.nz - z0));
+                if (near)
+                    hipLaunchKernelGGL((k_fwt3<NF, NT, true, XT, CT>), grid, dim3(256), 0, st, (const XT*)u,
+                                       (CT*)C, f.nx, f.ny, f.nz, z0, step, T);
+                else
+                    hipLaunchKernelGGL((k_fwt3<NF, NT, false, XT, CT>), grid, dim3(256), 0, st, (const XT*)u,
+                                       (CT*)C, f.nx, f.ny, f.nz, z0, step, T);
+            }
+        });
     else
         FL_FRAME_SWITCH(f.kind, hipLaunchKernelGGL((k_fwt2<NF, NT, XT, CT>), dim3(nblk(n)), dim3(256), 0, st,
                                                    (const XT*)u, (CT*)C, f.nx, f.ny, step, T));
