@@ -97,6 +97,7 @@ SIGNATURES = {
     "fl_adjoint": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_int, C.c_int, _vp]),
     "fl_siddon_trace": (C.c_int, [_vp, C.c_int, _ip, C.c_int, _ip, _fp, _ip]),
     "fl_calibrate_valu": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp]),
+    "fl_check_chord_sqrt": (C.c_int, [_vp, _vp]),
     "fl_framelet_num_bands": (C.c_int, [C.c_int, C.c_int]),
     "fl_framelet_forward": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_int,
                                       _vp, C.c_int, _vp]),

@@ -1,4 +1,5 @@
-// calib.hip -- VALU calibration kernel for counter-based rooflines.
+// calib.hip -- VALU calibration kernel for counter-based rooflines, and an
+// exhaustive self-check of the exact adjoint's square root.
 //
 // The projectors are bound by vector-ALU issue, not HBM.  To state what
 // fraction of the VALU peak they reach, the profiler's per-dispatch VALU
@@ -10,6 +11,7 @@
 
 #include "../../include/framelab_hip.h"
 #include "fl_capi_internal.h"
+#include "fl_chord.h"
 
 namespace {
 
@@ -38,7 +40,35 @@ __global__ __launch_bounds__(256) void k_valu_peak(float* __restrict__ out, int 
     out[i] = s;
 }
 
+// Every non-negative finite float (bit patterns 0 .. 0x7f7fffff, grid-
+// stride): does fl_sqrt_nonneg return the bits of the correctly rounded
+// sqrtf?  Mismatches are counted per thread and added to *bad.
+constexpr uint32_t kLastFinite = 0x7f7fffffu;
+
+__global__ __launch_bounds__(256) void k_check_sqrt(unsigned long long* __restrict__ bad)
+{
+    const uint32_t stride = gridDim.x * 256u;   // 2^24: p + stride stays below 2^32
+    unsigned long long n = 0;
+    for (uint32_t p = blockIdx.x * 256u + threadIdx.x; p <= kLastFinite; p += stride) {
+        const float s = __uint_as_float(p);
+        n += __float_as_uint(fl_sqrt_nonneg(s)) != __float_as_uint(sqrtf(s));
+    }
+    if (n) atomicAdd(bad, n);
+}
+
+__global__ void k_zero_u64(unsigned long long* __restrict__ v) { *v = 0; }
+
 }  // namespace
+
+extern "C" int fl_check_chord_sqrt(unsigned long long* d_bad, void* stream)
+{
+    if (!d_bad) return fl_fail(FL_ERR_INVALID, "null counter");
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, (hipStream_t)stream, d_bad);
+    hipLaunchKernelGGL(k_check_sqrt, dim3(65536), dim3(256), 0, (hipStream_t)stream, d_bad);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fl_hip_fail(e, "fl_check_chord_sqrt");
+    return fl_ok();
+}
 
 extern "C" int fl_calibrate_valu(float* d_out, int n_threads, int iters, int kind, void* stream)
 {

@@ -295,6 +295,13 @@ int fl_siddon_trace(fl_plan plan, int nrays, const int *rays, int cap, int *J, f
  * with. */
 int fl_calibrate_valu(float *d_out, int n_threads, int iters, int kind, void *stream);
 
+/* Self-check (no reference counterpart): the exact adjoint's chord length
+ * uses its own square root (the correctly rounded expansion without the
+ * rescaling of tiny inputs, which take sqrtf).  Compares it with sqrtf on
+ * every non-negative finite float and leaves the number of bit mismatches
+ * in the device counter *d_bad (zeroed by the call). */
+int fl_check_chord_sqrt(unsigned long long *d_bad, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
