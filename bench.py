@@ -17,6 +17,11 @@ Prints ONE JSON line.  value = proj+backproj GUPS = 2*N_vox*n_views /
                relative L2 difference to the FP32 one;
   roofline  -- the dominant kernel's algorithmic bytes per launch over its
                measured duration, against 8 TB/s;
+  framelet  -- 27-band 3-D framelet analysis and synthesis of the volume,
+               bandwidth against their algorithmic bytes;
+  batch     -- 64 uneven 2-D fan problems in persistent batched launches;
+  c4        -- the 512^3, 720-view, 1024x768 configuration in BF16 storage:
+               forward, exact and matched adjoint;
   cpu_baseline -- the reference's own single-threaded CPU kernels on a
                bounded sample of views.
 """
@@ -283,6 +288,43 @@ def framelet_leg(x, nvox, reps=3):
     return out
 
 
+def c4_leg():
+    """The largest configuration, 512^3 with 720 views and a 1024x768 panel,
+    in BF16 storage (FP32 arithmetic): one Siddon forward, one exact and one
+    matched adjoint, event-timed after a warm-up of each."""
+    import torch
+    from framelab_amd import geometry as G
+    from framelab_amd.projector import ADJ_EXACT, ADJ_MATCHED, Projector
+    from framelab_amd.phantoms import ellipsoids_3d_device
+    dev = torch.device("cuda:0")
+    N, nv, na, nb = 512, 720, 1024, 768
+    p = G.cone_para(na, nb, nv, N=(N, N, N), SO=4.0, OD=4.0, Ly=G.full_fan_width(), Lz=G.full_cone_height())
+    P = Projector(p)
+    x = ellipsoids_3d_device(N, count=20, seed=2, device=dev).to(torch.bfloat16)
+    y = torch.empty(P.sino_shape, dtype=torch.bfloat16, device=dev)
+    xo = torch.empty(P.vol_shape, dtype=torch.bfloat16, device=dev)
+    P.forward(x, out=y)
+    P.adjoint(y, out=xo, kind=ADJ_EXACT)
+    P.adjoint(y, out=xo, kind=ADJ_MATCHED)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record()
+    P.forward(x, out=y)
+    ev[1].record()
+    P.adjoint(y, out=xo, kind=ADJ_EXACT)
+    ev[2].record()
+    P.adjoint(y, out=xo, kind=ADJ_MATCHED)
+    ev[3].record()
+    torch.cuda.synchronize()
+    tA, tE, tM = (ev[k].elapsed_time(ev[k + 1]) for k in range(3))
+    upd = float(N) ** 3 * nv
+    out = {"workload": "cone-beam 512^3, 720 views, 1024x768 panel, BF16 storage, FP32 arithmetic",
+           "t_A_ms": tA, "t_At_exact_ms": tE, "t_At_matched_ms": tM,
+           "proj_backproj_gups_exact": 2 * upd / ((tA + tE) * 1e-3) / 1e9,
+           "proj_backproj_gups_matched": 2 * upd / ((tA + tM) * 1e-3) / 1e9}
+    log(f"bench: 512^3 BF16: A {tA:.1f} ms, At exact {tE:.1f} ms, matched {tM:.1f} ms")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -293,6 +335,7 @@ def main():
     ap.add_argument("--no-cgls", action="store_true", help="skip the CGLS leg")
     ap.add_argument("--cpu-views", type=int, default=3)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 2D leg")
+    ap.add_argument("--no-c4", action="store_true", help="skip the 512^3 BF16 leg")
     ap.add_argument("--profile", action="store_true",
                     help="counter run: one forward, one exact and one matched adjoint at 256^3, one "
                          "framelet analysis and synthesis (27 bands, FP32), then a "
@@ -455,6 +498,10 @@ def main():
             f"(loop {result['batch']['proj_backproj_gups_single_loop']:.1f}), cgls "
             f"{result['batch']['cgls_iters_per_s_batched']:.2f} it/s (loop "
             f"{result['batch']['cgls_iters_per_s_single_loop']:.2f})")
+
+    if not args.no_c4:
+        log("bench: 512^3 BF16")
+        result["c4"] = c4_leg()
 
     if not args.no_cpu:
         log("bench: cpu baseline")
