@@ -57,9 +57,10 @@ __device__ __forceinline__ int mwave_max(int v) { return __ockl_wfred_max_i32(v)
 // 3D, footprint-staged, z-column form: a workgroup is an 8 x 8 x (4*KZ)
 // voxel brick and each thread owns KZ z-adjacent voxels of one (x, y)
 // column.  Per view the workgroup stages in LDS the plane parameters of its
-// nine x planes and nine y planes for each footprint column, of its 4*KZ+1
-// z planes for each footprint row (each the forward's IEEE division, done
-// once per workgroup), the ray lengths |d| and the sinogram values; each
+// nine x planes and nine y planes for each footprint column, and for each
+// footprint row the entry and exit parameters of each of its 4*KZ slices'
+// z planes (each the forward's IEEE division, done once per workgroup),
+// the ray lengths |d| and the sinogram values; each
 // voxel then visits only the cells of its own shadow.  Views of each frame
 // in order, columns outer; FP32 accumulation.
 // Six workgroups per CU (measured optimum of the register budget).
@@ -70,7 +71,8 @@ __global__ __launch_bounds__(256, 6) void k_adj_matched_cone(const YT* __restric
     constexpr int BZ = 4 * KZ;
     __shared__ float t_ax[MF_A][9], t_ay[MF_A][9];
     __shared__ float t_dx[MF_A], t_dy[MF_A], t_rx[MF_A], t_ry[MF_A];
-    __shared__ float t_az[MF_B][BZ + 1], t_dz[MF_B], t_rz[MF_B];
+    __shared__ float2 t_zio[MF_B][BZ];   // per row and brick slice: z entry / exit plane parameters
+    __shared__ float t_dz[MF_B], t_rz[MF_B];
     __shared__ float t_L[MF_A * MF_B], t_y[MF_A * MF_B];
     __shared__ int red[2][4][4];
 
@@ -166,9 +168,16 @@ __global__ __launch_bounds__(256, 6) void k_adj_matched_cone(const YT* __restric
                 t_ax[ka][q] = fl_div_r((float)(ix0 + q - f.nx / 2) - x1, t_dx[ka], t_rx[ka]);
                 t_ay[ka][q] = fl_div_r((float)(iy0 + q - f.ny / 2) - y1, t_dy[ka], t_ry[ka]);
             }
-            for (int k = tid; k < nB * (BZ + 1); k += 256) {
-                const int kb = k / (BZ + 1), q = k % (BZ + 1);
-                t_az[kb][q] = fl_div_r((float)(iz0 + q - f.nz / 2) * f.dz0 - z1, t_dz[kb], t_rz[kb]);
+            // per row and brick slice the parameters of the slice's two z
+            // planes (the forward's quotients), ordered as entry and exit
+            // along the ray (+-inf for a ray parallel to them)
+            for (int k = tid; k < nB * BZ; k += 256) {
+                const int kb = k / BZ, q = k % BZ;
+                const float dz = t_dz[kb], rz = t_rz[kb];
+                const float lo = fl_div_r((float)(iz0 + q - f.nz / 2) * f.dz0 - z1, dz, rz);
+                const float hi = fl_div_r((float)(iz0 + q + 1 - f.nz / 2) * f.dz0 - z1, dz, rz);
+                t_zio[kb][q] = dz != 0.0f ? make_float2(dz > 0 ? lo : hi, dz > 0 ? hi : lo)
+                                          : make_float2(-FL_INF, FL_INF);
             }
             const uint32_t mA = fl_small_div_m((uint32_t)nA);   // k * nA < 2^18 in these tables
             for (int k = tid; k < nA * nB; k += 256) {
@@ -201,9 +210,8 @@ __global__ __launch_bounds__(256, 6) void k_adj_matched_cone(const YT* __restric
                         const float dz = t_dz[kb];
                         SidAxis az;
                         sid_axis_init(az, z1, dz, f.dz0, f.nz);
-                        const float z_lo = t_az[kb][lz], z_hi = t_az[kb][lz + 1];
-                        const float inz = az.on ? (dz > 0 ? z_lo : z_hi) : -FL_INF;
-                        const float outz = az.on ? (dz > 0 ? z_hi : z_lo) : FL_INF;
+                        const float2 zio = t_zio[kb][lz];
+                        const float inz = zio.x, outz = zio.y;
                         const int q = kb * nA + ka;
                         const float Si = matched_coef(inx, outx, iny, outy, inz, outz, ax, ay, az, ix, iy, iz,
                                                       dz_one, t_L[q]);
