@@ -63,9 +63,9 @@ __device__ __forceinline__ int mwave_max(int v) { return __ockl_wfred_max_i32(v)
 // the ray lengths |d| and the sinogram values; each
 // voxel then visits only the cells of its own shadow.  Views of each frame
 // in order, columns outer; FP32 accumulation.
-// Six workgroups per CU (measured optimum of the register budget).
+// Five workgroups per CU (measured optimum of the register budget).
 template <typename YT, typename XT, int KZ, bool DZ1>
-__global__ __launch_bounds__(256, 6) void k_adj_matched_cone(const YT* __restrict__ Y, XT* __restrict__ X,
+__global__ __launch_bounds__(256, 5) void k_adj_matched_cone(const YT* __restrict__ Y, XT* __restrict__ X,
                                                          FwdGeom f, AdjGeom g)
 {
     constexpr int BZ = 4 * KZ;
