@@ -268,21 +268,25 @@ __global__ __launch_bounds__(256) void k_iwt3s(const CT* __restrict__ C, XT* u, 
 
 // The same separable synthesis with the z reuse kept in registers, for
 // dilations 1 and 2 (compile-time STEP, so the z window is indexed
-// statically).  A workgroup owns the 64x16 tile for IZ_ZC output slices and
+// statically).  A workgroup owns the 64x16 tile for ZC output slices and
 // walks the band pairs (i, j): each thread loads, for its halo positions,
-// the NF bands (i, j, *) on all IZ_ZC + 2H input slices once and forms the
+// the NF bands (i, j, *) on all ZC + 2H input slices once and forms the
 // z pass of every output slice from registers (LDS, per slice); the x pass
 // is accumulated over j per slice (LDS), and after the last j the y pass
-// goes into the outputs.  Each coefficient is read (IZ_ZC + 2H) / IZ_ZC
+// goes into the outputs.  Each coefficient is read (ZC + 2H) / ZC
 // times instead of NT times, without depending on L2 to keep the slices
 // between the passes that share them.
-constexpr int IZ_ZC = 4;   // output slices per workgroup
 
 template <int NT, int STEP>
 struct IzGeom {
     static constexpr int CC = (NT - 1) / 2, H = CC * STEP;
-    static constexpr int PW = IW_TX + 2 * H, PH = IW_TY + 2 * H, P = PW * PH, WZ = IZ_ZC + 2 * H;
-    static constexpr size_t lds = ((size_t)IZ_ZC * P + (size_t)IZ_ZC * PH * IW_TX + P) * 4;
+    static constexpr int PW = IW_TX + 2 * H, PH = IW_TY + 2 * H, P = PW * PH;
+    static constexpr size_t lds_for(int zc) { return ((size_t)zc * P + (size_t)zc * PH * IW_TX + P) * 4; }
+    // output slices per workgroup: 8 where their LDS fits 80 KB (the linear
+    // frame at dilation 1: 1.25 reads per coefficient, two workgroups per
+    // CU, 5% faster than 4 slices at three per CU), else 4
+    static constexpr int ZC = lds_for(8) <= 80 * 1024 ? 8 : 4, WZ = ZC + 2 * H;
+    static constexpr size_t lds = lds_for(ZC);
 };
 
 template <int NF, int NT, int STEP, typename CT, typename XT>
@@ -290,24 +294,24 @@ __global__ __launch_bounds__(256) void k_iwt3z(const CT* __restrict__ C, XT* u, 
                                               float alpha, float beta, const float* base)
 {
     using G = IzGeom<NT, STEP>;
-    constexpr int CC = G::CC, H = G::H, PW = G::PW, PH = G::PH, P = G::P, WZ = G::WZ;
+    constexpr int CC = G::CC, H = G::H, PW = G::PW, PH = G::PH, P = G::P, WZ = G::WZ, ZC = G::ZC;
     constexpr int SX = PH * IW_TX;
-    __shared__ float tzs[IZ_ZC][P];    // z pass of the bands (i, j, *), per output slice
-    __shared__ float sxs[IZ_ZC][SX];   // x pass, summed over j, per output slice
+    __shared__ float tzs[ZC][P];    // z pass of the bands (i, j, *), per output slice
+    __shared__ float sxs[ZC][SX];   // x pass, summed over j, per output slice
     __shared__ int poff[P];            // halo position -> wrapped offset in a slice
     const size_t nxy = (size_t)nx * ny, nvox = nxy * nz;
     const int tilesx = (nx + IW_TX - 1) / IW_TX;
     const int x0 = (int)(blockIdx.x % tilesx) * IW_TX, y0 = (int)(blockIdx.x / tilesx) * IW_TY;
-    const int z0 = (int)blockIdx.y * IZ_ZC;
+    const int z0 = (int)blockIdx.y * ZC;
     const int tid = threadIdx.x;
     for (int p = tid; p < P; p += 256) poff[p] = wrapi(y0 - H + p / PW, ny) * nx + wrapi(x0 - H + p % PW, nx);
-    size_t zoff[WZ];   // input slices z0 - H .. z0 + IZ_ZC - 1 + H (wrapped)
+    size_t zoff[WZ];   // input slices z0 - H .. z0 + ZC - 1 + H (wrapped)
 #pragma unroll
     for (int w = 0; w < WZ; w++) zoff[w] = (size_t)wrapi(z0 - H + w, nz) * nxy;
     __syncthreads();
-    float acc[IZ_ZC][IW_OUT];
+    float acc[ZC][IW_OUT];
 #pragma unroll
-    for (int zz = 0; zz < IZ_ZC; zz++)
+    for (int zz = 0; zz < ZC; zz++)
 #pragma unroll
         for (int r = 0; r < IW_OUT; r++) acc[zz][r] = 0.0f;
 #pragma unroll 1
@@ -324,7 +328,7 @@ __global__ __launch_bounds__(256) void k_iwt3z(const CT* __restrict__ C, XT* u, 
                     for (int w = 0; w < WZ; w++) v[k][w] = fl_load(cb, zoff[w]);
                 }
 #pragma unroll
-                for (int zz = 0; zz < IZ_ZC; zz++) {
+                for (int zz = 0; zz < ZC; zz++) {
                     float t = 0.0f;
 #pragma unroll
                     for (int k = 0; k < NF; k++)
@@ -334,7 +338,7 @@ __global__ __launch_bounds__(256) void k_iwt3z(const CT* __restrict__ C, XT* u, 
                 }
             }
             __syncthreads();
-            for (int q = tid; q < IZ_ZC * SX; q += 256) {
+            for (int q = tid; q < ZC * SX; q += 256) {
                 const int zz = q / SX, q2 = q % SX;
                 const float* row = &tzs[zz][(q2 / IW_TX) * PW + q2 % IW_TX + H];
                 float s = 0.0f;
@@ -345,7 +349,7 @@ __global__ __launch_bounds__(256) void k_iwt3z(const CT* __restrict__ C, XT* u, 
             __syncthreads();
         }
 #pragma unroll
-        for (int zz = 0; zz < IZ_ZC; zz++)
+        for (int zz = 0; zz < ZC; zz++)
 #pragma unroll
             for (int r = 0; r < IW_OUT; r++) {
                 const int o = tid + 256 * r, ox = o % IW_TX, oy = o / IW_TX;
@@ -357,7 +361,7 @@ __global__ __launch_bounds__(256) void k_iwt3z(const CT* __restrict__ C, XT* u, 
             }
     }
 #pragma unroll
-    for (int zz = 0; zz < IZ_ZC; zz++) {
+    for (int zz = 0; zz < ZC; zz++) {
         const int z = z0 + zz;
         if (z >= nz) break;
 #pragma unroll
@@ -638,17 +642,20 @@ static hipError_t iwt_t(const FlFrameDesc& f, const void* C, void* u, int level,
     if (f.dim == 3)
         FL_FRAME_SWITCH(f.kind, {
             // dilations 1 and 2: the z-window tiles (while their LDS fits
-            // 48 KB); then the separable tiles while the halo tile fits;
+            // 80 and 48 KB); then the separable tiles while the halo tile fits;
             // the per-voxel gather beyond
-            const dim3 zgrid((unsigned)(((f.nx + IW_TX - 1) / IW_TX) * ((f.ny + IW_TY - 1) / IW_TY)),
-                             (unsigned)((f.nz + IZ_ZC - 1) / IZ_ZC));
-            if (step == 1 && IzGeom<NT, 1>::lds <= 48 * 1024) {
+            const unsigned ztiles = (unsigned)(((f.nx + IW_TX - 1) / IW_TX) * ((f.ny + IW_TY - 1) / IW_TY));
+            if (step == 1 && IzGeom<NT, 1>::lds <= 80 * 1024) {
+                const int zc = (IzGeom<NT, 1>::ZC);
+                const dim3 zgrid(ztiles, (unsigned)((f.nz + zc - 1) / zc));
                 hipLaunchKernelGGL((k_iwt3z<NF, NT, 1, CT, XT>), zgrid, dim3(256), 0, st, (const CT*)C, (XT*)u, f.nx,
                                    f.ny, f.nz, T, alpha, beta, base);
                 return hipGetLastError();
             }
             if constexpr (IzGeom<NT, 2>::lds <= 48 * 1024) {
                 if (step == 2) {
+                    const int zc = (IzGeom<NT, 2>::ZC);
+                    const dim3 zgrid(ztiles, (unsigned)((f.nz + zc - 1) / zc));
                     hipLaunchKernelGGL((k_iwt3z<NF, NT, 2, CT, XT>), zgrid, dim3(256), 0, st, (const CT*)C, (XT*)u,
                                        f.nx, f.ny, f.nz, T, alpha, beta, base);
                     return hipGetLastError();
