@@ -63,6 +63,18 @@ __device__ __forceinline__ uint16_t fl_f32_to_bf16(float f)
 __device__ __forceinline__ void fl_store(float* p, size_t i, float v) { p[i] = v; }
 __device__ __forceinline__ void fl_store(bf16_t* p, size_t i, float v) { p[i].v = fl_f32_to_bf16(v); }
 __device__ __forceinline__ void fl_store(f16_t* p, size_t i, float v) { p[i].v = (_Float16)v; }
+// Streaming (nontemporal) stores for outputs written once and not re-read
+// by the same kernel: 27 band streams per wave write-combine better when
+// they do not displace L2 lines.
+__device__ __forceinline__ void fl_store_nt(float* p, size_t i, float v) { __builtin_nontemporal_store(v, p + i); }
+__device__ __forceinline__ void fl_store_nt(bf16_t* p, size_t i, float v)
+{
+    __builtin_nontemporal_store(fl_f32_to_bf16(v), &p[i].v);
+}
+__device__ __forceinline__ void fl_store_nt(f16_t* p, size_t i, float v)
+{
+    __builtin_nontemporal_store((_Float16)v, &p[i].v);
+}
 
 // Map a linear workgroup id so that consecutive logical tiles land on the
 // same XCD (the dispatcher deals workgroups round-robin over 8 XCDs, each

@@ -97,7 +97,8 @@ __device__ __forceinline__ void bands3(const FlTaps& T, const float (&nb)[NT][NT
 }
 
 // Workgroups of 64 x 4 (x, y) voxels of one slice: coordinates without
-// divides.
+// divides.  Band stores are nontemporal (0.35 ms instead of 0.44 ms for 27
+// FP32 bands of 256^3).
 template <int NF, int NT, bool NEAR, typename XT, typename CT>
 __global__ __launch_bounds__(256) void k_fwt3(const XT* __restrict__ u, CT* __restrict__ C, int nx, int ny,
                                              int nz, int z0, int step, FlTaps T)
@@ -113,7 +114,7 @@ __global__ __launch_bounds__(256) void k_fwt3(const XT* __restrict__ u, CT* __re
     float out[NF * NF * NF];
     bands3<NF, NT>(T, nb, out);
 #pragma unroll
-    for (int b = 0; b < NF * NF * NF; b++) fl_store(C, (size_t)b * nvox + idx, out[b]);
+    for (int b = 0; b < NF * NF * NF; b++) fl_store_nt(C, (size_t)b * nvox + idx, out[b]);
 }
 
 // u = alpha * sum_b corr(R-stencil of b, C_b) + beta * base (base may be null)
