@@ -387,12 +387,14 @@ def main():
         dst = torch.empty_like(src)
         dst.copy_(src)
         # VALU calibration: 8192 workgroups of FP32 FMA chains, then of integer chains
-        from framelab_amd import _lib
         import ctypes as C
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from diag import diag
         out = torch.empty(8192 * 256, dtype=torch.float32, device=dev)
         for kind in (0, 1):
-            _lib.check(_lib.lib().fl_calibrate_valu(C.c_void_p(out.data_ptr()), out.numel(), 5000, kind,
-                                                    C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+            diag.check(diag.lib().fl_calibrate_valu(C.c_void_p(out.data_ptr()), out.numel(), 5000, kind,
+                                                    C.c_void_p(torch.cuda.current_stream().cuda_stream)),
+                       "fl_calibrate_valu")
         torch.cuda.synchronize()
         log("bench: profile run done (forward, exact adjoint, matched adjoint, framelet analysis and "
             "synthesis, 1 GiB copy, VALU calibration)")

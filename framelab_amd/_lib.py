@@ -96,8 +96,10 @@ SIGNATURES = {
                                      _fp, _fp, _fp, _fp]),
     "fl_adjoint": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_int, C.c_int, _vp]),
     "fl_siddon_trace": (C.c_int, [_vp, C.c_int, _ip, C.c_int, _ip, _fp, _ip]),
-    "fl_calibrate_valu": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp]),
-    "fl_check_chord_sqrt": (C.c_int, [_vp, _vp]),
+    "fl_forward_f64": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "fl_adjoint_f64": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "fl_cgls_f64": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_int, C.c_double, C.POINTER(C.c_double),
+                              C.POINTER(C.c_int), _vp]),
     "fl_framelet_num_bands": (C.c_int, [C.c_int, C.c_int]),
     "fl_framelet_forward": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_int,
                                       _vp, C.c_int, _vp]),
@@ -105,11 +107,16 @@ SIGNATURES = {
                                       _vp, C.c_int, _vp, _vp]),
     "fl_cgls": (C.c_int, [_vp, _vp, _vp, C.c_float, C.c_int, C.c_double, C.c_int,
                           C.POINTER(C.c_double), C.POINTER(C.c_int), _vp]),
-    "fl_split_bregman": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_float, C.c_float, C.c_int, C.c_int,
+    "fl_split_bregman": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.c_float, C.c_float, C.c_int, C.c_int,
                                    C.c_double, C.c_int, C.POINTER(C.c_double), _vp]),
+    "fl_vec_cgls_init": (C.c_int, [C.c_longlong, _vp, _vp, _vp, C.c_float, _vp, _vp, C.POINTER(C.c_double), _vp]),
+    "fl_vec_cgls_pq": (C.c_int, [C.c_longlong, _vp, _vp, C.c_float, C.POINTER(C.c_double), _vp]),
+    "fl_vec_cgls_update": (C.c_int, [C.c_longlong, _vp, _vp, _vp, _vp, C.c_double, C.POINTER(C.c_double), _vp]),
+    "fl_vec_cgls_dir": (C.c_int, [C.c_longlong, _vp, _vp, C.c_double, _vp]),
+    "fl_framelet_shrink": (C.c_int, [C.c_int, C.c_int, C.c_longlong, _vp, C.c_int, C.c_float, _vp]),
     "fl_grad_data": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(C.c_double), C.c_int, _vp]),
-    "fl_grad_framelet": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_float, C.c_float,
-                                   C.c_int, _vp, _vp, C.POINTER(C.c_double), _vp]),
+    "fl_grad_framelet": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_float,
+                                   C.c_float, C.c_int, _vp, _vp, C.POINTER(C.c_double), _vp]),
     "fl_plan_autotune": (C.c_int, [_vp, _ip, _fp]),
     "fl_plan_set_tile": (C.c_int, [_vp, C.c_int]),
     "fl_forward_norm": (C.c_int, [_vp, _vp, _vp, _vp]),

@@ -71,7 +71,8 @@ static int upload(T** dst, const T* src, size_t n, const char* what)
 static void plan_free_arrays(fl_plan_s* p)
 {
     void* ptrs[] = {p->d_cs, p->d_sn, p->d_sdz, p->d_ydet, p->d_zdet, p->d_cphi, p->d_sphi,
-                    p->d_cdet, p->d_sdet, p->d_slot_view, p->d_view_frame, p->d_idY, p->d_Nv, p->d_bricks,
+                    p->d_cdet, p->d_sdet, p->d_slot_view, p->d_view_frame, p->d_idY, p->d_idS, p->d_Nv,
+                    p->d_bricks,
                     p->d_fb_px, p->d_fb_py, p->d_fb_pd, p->d_cdet2, p->d_sdet2, p->d_line_q,
                     p->d_ray_q};
     for (void* q : ptrs)
@@ -208,6 +209,26 @@ extern "C" int fl_plan_create(fl_plan* out, const fl_geometry* g)
         for (int v = 0; v < g->nv; v++) idY[v] = v;
     }
     if (idY.empty()) idY.push_back(0);
+    // The adjoint reads the frame-list entry (it, j) from the slot the
+    // forward writes its view to: the n-th listing of view v (frames in
+    // order, then list order) reads the n-th slot holding v.  Without
+    // slot_view, slot k holds view k.  (The reference GPU wrappers fill and
+    // read their sinograms in this frame-list order,
+    // Ax_cone_mf_gpu_new.cu:500-517, Atx_cone_mf_gpu_new.cu:140-150.)
+    std::vector<int> idS(idY);
+    bool slots_ok = true;
+    if (g->slot_view) {
+        std::vector<std::vector<int>> holders(g->nv);
+        for (int k = 0; k < g->nv; k++) holders[slot_view[k]].push_back(k);
+        std::vector<size_t> used(g->nv, 0);
+        for (int it = 0; it < g->nt; it++)
+            for (int j = 0; j < Nv[it]; j++) {
+                const size_t e = (size_t)it * tmp + j;
+                const int v = idY[e];
+                if (used[v] < holders[v].size()) idS[e] = holders[v][used[v]++];
+                else slots_ok = false;
+            }
+    }
 
     // forward trig: cosf/sinf of the single-precision angles, as the
     // reference's C++ forward evaluates (float)cos(sd_phi[iv])
@@ -247,6 +268,7 @@ extern "C" int fl_plan_create(fl_plan* out, const fl_geometry* g)
     p->nt = g->nt;
     p->nv = g->nv;
     p->h_sd_phi.assign(g->sd_phi, g->sd_phi + g->nv);
+    p->slots_ok = slots_ok;
     int rc = FL_OK;
     if ((rc = upload(&p->d_cs, cs.data(), cs.size(), "cos table")) ||
         (rc = upload(&p->d_sn, sn.data(), sn.size(), "sin table")) ||
@@ -260,6 +282,7 @@ extern "C" int fl_plan_create(fl_plan* out, const fl_geometry* g)
         (rc = upload(&p->d_slot_view, slot_view.data(), slot_view.size(), "slot_view")) ||
         (rc = upload(&p->d_view_frame, view_frame.data(), view_frame.size(), "id_X")) ||
         (rc = upload(&p->d_idY, idY.data(), idY.size(), "id_Y")) ||
+        (rc = upload(&p->d_idS, idS.data(), idS.size(), "slot per frame-list entry")) ||
         (rc = upload(&p->d_Nv, Nv.data(), Nv.size(), "Nv"))) {
         plan_free_arrays(p);
         delete p;
@@ -272,7 +295,9 @@ extern "C" int fl_plan_create(fl_plan* out, const fl_geometry* g)
         delete p;
         return rc;
     }
-    if (!cone) p->h_ydet.assign(g->y_det, g->y_det + g->na);
+    p->h_ydet.assign(g->y_det, g->y_det + g->na);
+    p->h_zdet = zdet;
+    p->h_sdz = sdz;
     if (cone) {
         // the forward gathers from a bricked copy of its input (8x8x1 bricks,
         // padded to whole bricks; 4-byte elements cover every storage type)
@@ -298,7 +323,8 @@ extern "C" int fl_plan_create(fl_plan* out, const fl_geometry* g)
 
     AdjGeom& a = p->ag;
     a.cphi = p->d_cphi; a.sphi = p->d_sphi; a.cdet = p->d_cdet; a.sdet = p->d_sdet;
-    a.sdz = p->d_sdz; a.ydet = p->d_ydet; a.zdet = p->d_zdet; a.id_Y = p->d_idY; a.Nv = p->d_Nv;
+    a.sdz = p->d_sdz; a.ydet = p->d_ydet; a.zdet = p->d_zdet; a.id_Y = p->d_idY; a.id_S = p->d_idS;
+    a.Nv = p->d_Nv;
     a.SO = g->SO; a.OD = g->OD; a.scale = g->scale; a.dy_det = g->dy_det; a.y_os = g->y_os;
     a.dz_det = g->dz_det; a.dz = cone ? g->dz : 1.0f;
     a.nx = g->nx; a.ny = g->ny; a.nz = cone ? g->nz : 1; a.na = g->na; a.nb = g->nb; a.nt = g->nt;
@@ -362,6 +388,7 @@ extern "C" int fl_plan_destroy(fl_plan p)
 {
     if (!p) return fl_ok();
     fl_plan_release_workspace(p);
+    fl_plan_release_f64(p);
     plan_free_arrays(p);
     delete p;
     return fl_ok();
@@ -539,6 +566,9 @@ extern "C" int fl_adjoint(fl_plan p, const void* d_y, int yt, void* d_x, int xt,
 {
     if (!p || !d_x || !d_y) return fl_fail(FL_ERR_INVALID, "null plan or buffer");
     if (!dtype_ok(xt) || !dtype_ok(yt)) return fl_fail(FL_ERR_INVALID, "bad dtype");
+    if (!p->slots_ok)
+        return fl_fail(FL_ERR_UNSUPPORTED, "a frame lists a view more often than the forward's output slots "
+                                           "hold it (slot_view), so the adjoint has no slot to read");
     if (p->det_dev > FL_DET_DEV_MAX)
         return fl_fail(FL_ERR_UNSUPPORTED,
                        "detector coordinates stray %.3g cells from a uniform grid; the voxel-driven adjoints "
@@ -810,8 +840,16 @@ extern "C" void Atx_cone_mf_gpu_new(float* X, float* y, float* cos_phi, float* s
     g.cos_phi = cos_phi; g.sin_phi = sin_phi; g.id_Y = id_Y; g.Nv = Nv; g.tmp_size = tmp_size;
     std::vector<int> idX, slots;
     if (id_Y && Nv) {
+        // the reference wrapper reads y in frame-list order (slot id holds
+        // the id-th listed view, Atx_cone_mf_gpu_new.cu:140-150), the order
+        // the cone forward drop-ins write
         frames_to_views(nt, nv, id_Y, Nv, tmp_size, idX, slots);
+        if ((int)slots.size() != nv) {
+            fl_fail(FL_ERR_INVALID, "id_Y/Nv list %zu views, expected nv=%d", slots.size(), nv);
+            return;
+        }
         g.id_X = idX.data();
+        g.slot_view = slots.data();
     }
     run_adjoint_host(g, y, X);
 }

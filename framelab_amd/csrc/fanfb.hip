@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) void k_adj_fan_fb(const float* __restrict__ Y,
     const int nvf = g.Nv[it];
     for (int j = 0; j < nvf; j++) {
         const int iv = g.id_Y[it * g.tmp_size + j];
-        const float* __restrict__ y = Y + (size_t)iv * nd;
+        const float* __restrict__ y = Y + (size_t)g.id_S[it * g.tmp_size + j] * nd;
         const float c = g.cphi[iv], s = g.sphi[iv];
         const float xr = c * xc + s * yc;
         const float yr = -s * xc + c * yc;

@@ -20,6 +20,8 @@ struct fl_plan_s {
     float *d_cs = nullptr, *d_sn = nullptr, *d_sdz = nullptr, *d_ydet = nullptr, *d_zdet = nullptr;
     float *d_cphi = nullptr, *d_sphi = nullptr, *d_cdet = nullptr, *d_sdet = nullptr;
     int *d_slot_view = nullptr, *d_view_frame = nullptr, *d_idY = nullptr, *d_Nv = nullptr;
+    int* d_idS = nullptr;         // adjoint: sinogram slot per frame-list entry (AdjGeom::id_S)
+    bool slots_ok = true;         // every frame-list view has a forward output slot to read
     void* d_bricks = nullptr;   // cone: bricked copy of the forward input (nt frames, 8x8 bricks, 4-byte)
     // forwards share d_bricks: each one waits for the previous one (on
     // whatever stream it ran) through this event, recorded after it
@@ -32,11 +34,15 @@ struct fl_plan_s {
     float *d_cdet2 = nullptr, *d_sdet2 = nullptr;
     float2* d_line_q = nullptr;   // fan: per (view, cell) line-chord quotients
     float4* d_ray_q = nullptr;    // fan: per (view, cell) ray reciprocals and length
-    std::vector<float> h_ydet;
+    std::vector<float> h_ydet, h_zdet, h_sdz;
     // largest distance, in cells, of a detector coordinate from the uniform
     // grid through the first and last cell: the adjoints' window maps assume
     // a uniform pitch
     double det_dev = 0.0;
+    // FP64 projector pair: geometry in double (cos/sin of the double angle,
+    // source heights, detector coordinates), built on first use
+    double *d_c64 = nullptr, *d_s64 = nullptr, *d_sdz64 = nullptr, *d_ydet64 = nullptr, *d_zdet64 = nullptr;
+    std::mutex f64_mu;
     // lazily allocated workspace (solvers, gradients)
     void* ws = nullptr;
     size_t ws_bytes = 0;
@@ -65,3 +71,4 @@ hipError_t fl_launch_adj_cone(const void* Y, int yt, void* X, int xt, const AdjG
 hipError_t fl_launch_adj_matched(const void* Y, int yt, void* X, int xt, const fl_plan_s* p, hipStream_t st);
 hipError_t fl_launch_fwd_fan_fb(const float* X, float* Y, const fl_plan_s* p, hipStream_t st);
 hipError_t fl_launch_adj_fan_fb(const float* Y, float* X, const fl_plan_s* p, hipStream_t st);
+void fl_plan_release_f64(fl_plan_s* p);

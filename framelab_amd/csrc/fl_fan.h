@@ -124,7 +124,7 @@ __device__ __forceinline__ float fl_fan_adj_pixel(const YT* __restrict__ Y, cons
     const int nvf = g.Nv[it];
     for (int j = 0; j < nvf; j++) {
         const int iv = g.id_Y[it * g.tmp_size + j];
-        const YT* __restrict__ y = Y + (size_t)iv * nd;
+        const YT* __restrict__ y = Y + (size_t)g.id_S[it * g.tmp_size + j] * nd;
         const float c = g.cphi[iv], s = g.sphi[iv];
         const float xr = c * xc + s * yc;
         const float yr = -s * xc + c * yc;
@@ -203,7 +203,7 @@ __device__ __forceinline__ float fl_fan_matched_pixel(const YT* __restrict__ Y, 
     const int nvf = g.Nv[it];
     for (int j = 0; j < nvf; j++) {
         const int iv = g.id_Y[it * g.tmp_size + j];
-        const YT* __restrict__ y = Y + (size_t)iv * nd;
+        const YT* __restrict__ y = Y + (size_t)g.id_S[it * g.tmp_size + j] * nd;
         const float c = f.cs[iv], s = f.sn[iv];
         const float xr = c * xc + s * yc, yr = -s * xc + c * yc;
         const float mag = SD / (xr + f.SO);

@@ -96,16 +96,19 @@ __device__ __forceinline__ float fl_synth2(const CT* __restrict__ C, size_t band
 
 // Split-Bregman coefficient update at one pixel (solveSplitBregman.m:28-36):
 // t = v + Wu, alpha = shrink(t) (low-pass copied), v <- v + (Wu - alpha),
-// d = alpha - v.  Returns the pixel's share of |Wu - alpha|^2.
+// d = alpha - v.  Returns the pixel's share of |Wu - alpha|^2.  lp (may be
+// null) receives the low-pass band of Wu, the input of the next level.
 template <int NF, int NT, typename CT>
 __device__ __forceinline__ double fl_bregman2_pixel(const float* __restrict__ u, CT* __restrict__ V,
                                                     CT* __restrict__ Dc, size_t band_stride, size_t idx, int nx,
-                                                    int ny, int x, int y, int step, const FlTaps& T, float tau)
+                                                    int ny, int x, int y, int step, const FlTaps& T, float tau,
+                                                    float* __restrict__ lp = nullptr)
 {
     float nb[NT][NT];
     fl_gather2<NT>(u, nx, ny, x, y, step, nb);
     float w[NF * NF];
     fl_bands2<NF, NT>(T, nb, w);
+    if (lp) lp[idx] = w[0];
     double r2 = 0.0;
 #pragma unroll
     for (int b = 0; b < NF * NF; b++) {

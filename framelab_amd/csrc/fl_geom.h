@@ -31,6 +31,8 @@ struct AdjGeom {
     const float* ydet;
     const float* zdet;
     const int* id_Y;        // frame -> views, tmp_size per frame
+    const int* id_S;        // sinogram slot holding the j-th view of frame it (id_Y's layout):
+                            // the slot the forward writes that view to
     const int* Nv;          // views per frame
     float SO, OD, scale, dy_det, y_os, dz_det, dz;
     int nx, ny, nz, na, nb, nt, tmp_size;

@@ -417,11 +417,15 @@ __global__ __launch_bounds__(256) void k_iwt2(const CT* __restrict__ C, XT* u, i
 //   v <- v + (Wu - alpha);  d = alpha - v   (consumed by b = Atf + mu W^T d).
 // Analysis, shrink and both updates happen in one pass; Wu and alpha are
 // never stored.  resid accumulates per-block FP64 partials of |Wu - alpha|^2
-// (the constraint residual used as the outer stopping test).
+// (the constraint residual used as the outer stopping test).  One launch per
+// level (dilation `step`); `lp`, when given, receives the level's low-pass
+// band of Wu, the input of the next level (FraDecMultiLevel3D.m:16-19).
 //
 // Smoothed l1 (gradient of lam * sum psi(Wu), psi(t) = sqrt(t^2 + eps)):
 //   g_b = lam * psi'(Wu_b) = lam * Wu_b / sqrt(Wu_b^2 + eps), low-pass band 0
-//   unless with_lowpass; loss partials of lam * sum psi.
+//   unless with_lowpass; loss partials of lam * sum psi.  A level below the
+//   top (lp_pass) stores its low-pass band of Wu itself in band 0 instead:
+//   the next level's input, later overwritten by that level's synthesis.
 
 template <int NB>
 __device__ __forceinline__ void block_reduce_store(double v, double* __restrict__ part)
@@ -440,7 +444,8 @@ __device__ __forceinline__ void block_reduce_store(double v, double* __restrict_
 template <int NF, int NT, typename CT>
 __global__ __launch_bounds__(256) void k_bregman3(const float* __restrict__ u, CT* __restrict__ V,
                                                  CT* __restrict__ Dc, int nx, int ny, int nz, int step,
-                                                 FlTaps T, float tau, double* __restrict__ part)
+                                                 FlTaps T, float tau, double* __restrict__ part,
+                                                 float* __restrict__ lp)
 {
     constexpr int NB = NF * NF * NF;
     const size_t nvox = (size_t)nx * ny * nz;
@@ -452,6 +457,7 @@ __global__ __launch_bounds__(256) void k_bregman3(const float* __restrict__ u, C
         gather3<NF, NT>(u, nx, ny, nz, x, y, z, step, nb);
         float wu[NB];
         bands3<NF, NT>(T, nb, wu);
+        if (lp) lp[idx] = wu[0];
 #pragma unroll
         for (int b = 0; b < NB; b++) {
             const size_t o = (size_t)b * nvox + idx;
@@ -471,14 +477,14 @@ __global__ __launch_bounds__(256) void k_bregman3(const float* __restrict__ u, C
 template <int NF, int NT, typename CT>
 __global__ __launch_bounds__(256) void k_bregman2(const float* __restrict__ u, CT* __restrict__ V,
                                                  CT* __restrict__ Dc, int nx, int ny, int step, FlTaps T,
-                                                 float tau, double* __restrict__ part)
+                                                 float tau, double* __restrict__ part, float* __restrict__ lp)
 {
     const size_t npix = (size_t)nx * ny;
     const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
     double r2 = 0.0;
     if (idx < npix) {
         const int x = (int)(idx % nx), y = (int)(idx / nx);
-        r2 = fl_bregman2_pixel<NF, NT>(u, V, Dc, npix, idx, nx, ny, x, y, step, T, tau);
+        r2 = fl_bregman2_pixel<NF, NT>(u, V, Dc, npix, idx, nx, ny, x, y, step, T, tau, lp);
     }
     block_reduce_store<NF * NF>(r2, part);
 }
@@ -486,7 +492,7 @@ __global__ __launch_bounds__(256) void k_bregman2(const float* __restrict__ u, C
 template <int NF, int NT, typename CT>
 __global__ __launch_bounds__(256) void k_l1grad3(const float* __restrict__ u, CT* __restrict__ G, int nx, int ny,
                                                 int nz, int step, FlTaps T, float lam, float eps, int with_lp,
-                                                double* __restrict__ part)
+                                                int lp_pass, double* __restrict__ part)
 {
     constexpr int NB = NF * NF * NF;
     const size_t nvox = (size_t)nx * ny * nz;
@@ -501,7 +507,9 @@ __global__ __launch_bounds__(256) void k_l1grad3(const float* __restrict__ u, CT
 #pragma unroll
         for (int b = 0; b < NB; b++) {
             float g = 0.0f;
-            if (b != 0 || with_lp) {
+            if (b == 0 && lp_pass) {
+                g = wu[0];
+            } else if (b != 0 || with_lp) {
                 const float r = sqrtf(__builtin_fmaf(wu[b], wu[b], eps));
                 g = lam * (wu[b] / r);
                 loss += (double)lam * (double)r;
@@ -514,7 +522,7 @@ __global__ __launch_bounds__(256) void k_l1grad3(const float* __restrict__ u, CT
 
 template <int NF, int NT, typename CT>
 __global__ __launch_bounds__(256) void k_l1grad2(const float* __restrict__ u, CT* __restrict__ G, int nx, int ny,
-                                                int step, FlTaps T, float lam, float eps, int with_lp,
+                                                int step, FlTaps T, float lam, float eps, int with_lp, int lp_pass,
                                                 double* __restrict__ part)
 {
     constexpr int CC = (NT - 1) / 2;
@@ -547,7 +555,9 @@ __global__ __launch_bounds__(256) void k_l1grad2(const float* __restrict__ u, CT
                 for (int ky = 0; ky < NT; ky++) w = __builtin_fmaf(T.D[j][ky], tx[ky], w);
                 const int b = i + NF * j;
                 float g = 0.0f;
-                if (b != 0 || with_lp) {
+                if (b == 0 && lp_pass) {
+                    g = w;
+                } else if (b != 0 || with_lp) {
                     const float r = sqrtf(__builtin_fmaf(w, w, eps));
                     g = lam * (w / r);
                     loss += (double)lam * (double)r;
@@ -706,40 +716,44 @@ hipError_t fl_launch_iwt(const FlFrameDesc& f, const void* C, int ct, void* u, i
 }
 
 template <typename CT>
-static hipError_t bregman_t(const FlFrameDesc& f, const float* u, void* V, void* D, float tau, double* part,
-                            hipStream_t st)
+static hipError_t bregman_t(const FlFrameDesc& f, const float* u, void* V, void* D, float tau, int level,
+                            float* lp, double* part, hipStream_t st)
 {
     const FlTaps T = fl_make_taps(f.kind);
     const size_t n = (size_t)f.nx * f.ny * (f.dim == 3 ? f.nz : 1);
+    const int step = 1 << (level - 1);
     if (f.dim == 3)
         FL_FRAME_SWITCH(f.kind, hipLaunchKernelGGL((k_bregman3<NF, NT, CT>), dim3(nblk(n)), dim3(256), 0, st, u,
-                                                   (CT*)V, (CT*)D, f.nx, f.ny, f.nz, 1, T, tau, part));
+                                                   (CT*)V, (CT*)D, f.nx, f.ny, f.nz, step, T, tau, part, lp));
     else
         FL_FRAME_SWITCH(f.kind, hipLaunchKernelGGL((k_bregman2<NF, NT, CT>), dim3(nblk(n)), dim3(256), 0, st, u,
-                                                   (CT*)V, (CT*)D, f.nx, f.ny, 1, T, tau, part));
+                                                   (CT*)V, (CT*)D, f.nx, f.ny, step, T, tau, part, lp));
     return hipGetLastError();
 }
 
-hipError_t fl_launch_bregman(const FlFrameDesc& f, const float* u, void* V, void* D, int ct, float tau,
-                             double* part, hipStream_t st)
+hipError_t fl_launch_bregman(const FlFrameDesc& f, const float* u, void* V, void* D, int ct, float tau, int level,
+                             float* lp, double* part, hipStream_t st)
 {
-    if (ct == FL_F32) return bregman_t<float>(f, u, V, D, tau, part, st);
-    if (ct == FL_BF16) return bregman_t<bf16_t>(f, u, V, D, tau, part, st);
-    if (ct == FL_F16) return bregman_t<f16_t>(f, u, V, D, tau, part, st);
+    if (level < 1 || level > 16) return hipErrorInvalidValue;
+    if (ct == FL_F32) return bregman_t<float>(f, u, V, D, tau, level, lp, part, st);
+    if (ct == FL_BF16) return bregman_t<bf16_t>(f, u, V, D, tau, level, lp, part, st);
+    if (ct == FL_F16) return bregman_t<f16_t>(f, u, V, D, tau, level, lp, part, st);
     return hipErrorInvalidValue;
 }
 
 hipError_t fl_launch_l1grad(const FlFrameDesc& f, const float* u, float* G, float lam, float eps, int with_lp,
-                            double* part, hipStream_t st)
+                            int level, int lp_pass, double* part, hipStream_t st)
 {
+    if (level < 1 || level > 16) return hipErrorInvalidValue;
     const FlTaps T = fl_make_taps(f.kind);
     const size_t n = (size_t)f.nx * f.ny * (f.dim == 3 ? f.nz : 1);
+    const int step = 1 << (level - 1);
     if (f.dim == 3)
         FL_FRAME_SWITCH(f.kind, hipLaunchKernelGGL((k_l1grad3<NF, NT, float>), dim3(nblk(n)), dim3(256), 0, st, u,
-                                                   G, f.nx, f.ny, f.nz, 1, T, lam, eps, with_lp, part));
+                                                   G, f.nx, f.ny, f.nz, step, T, lam, eps, with_lp, lp_pass, part));
     else
         FL_FRAME_SWITCH(f.kind, hipLaunchKernelGGL((k_l1grad2<NF, NT, float>), dim3(nblk(n)), dim3(256), 0, st, u,
-                                                   G, f.nx, f.ny, 1, T, lam, eps, with_lp, part));
+                                                   G, f.nx, f.ny, step, T, lam, eps, with_lp, lp_pass, part));
     return hipGetLastError();
 }
 

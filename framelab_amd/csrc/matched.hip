@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256, 5) void k_adj_matched_cone(const YT* __restric
     const int nvf = g.Nv[it];
     for (int j = 0; j < nvf; j++) {
         const int iv = g.id_Y[it * g.tmp_size + j];
-        const YT* __restrict__ y = Y + (size_t)iv * g.nd;
+        const YT* __restrict__ y = Y + (size_t)g.id_S[it * g.tmp_size + j] * g.nd;
         const float c = f.cs[iv], s = f.sn[iv];
         const float sz = f.sdz[iv];
         int alo = na, ahi = -1, ublo = nb, ubhi = -1;

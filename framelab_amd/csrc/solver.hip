@@ -194,6 +194,53 @@ __global__ __launch_bounds__(256) void k_sum_to(const double* __restrict__ part,
     if (threadIdx.x == 0) *out = v;
 }
 
+// CGLS vector steps with host-supplied scalars (CGLS over caller-supplied
+// operators): the same element arithmetic as the plan solver's kernels.
+__global__ __launch_bounds__(256) void k_gv_pq(float* __restrict__ q, const float* __restrict__ p, float lam, size_t n,
+                                              double* __restrict__ part)
+{
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const float qi = __builtin_fmaf(lam, p[i], q[i]);
+        q[i] = qi;
+        acc += (double)p[i] * (double)qi;
+    }
+    store_partial(acc, part);
+}
+
+__global__ __launch_bounds__(256) void k_gv_update(float* __restrict__ u, float* __restrict__ r,
+                                                  const float* __restrict__ p, const float* __restrict__ q, float a,
+                                                  size_t n, double* __restrict__ part)
+{
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        u[i] = __builtin_fmaf(a, p[i], u[i]);
+        const float ri = __builtin_fmaf(a, q[i], r[i]);
+        r[i] = ri;
+        acc += (double)ri * (double)ri;
+    }
+    store_partial(acc, part);
+}
+
+__global__ __launch_bounds__(256) void k_gv_dir(float* __restrict__ p, const float* __restrict__ r, float bt, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        p[i] = __builtin_fmaf(bt, p[i], -r[i]);
+}
+
+// soft threshold of every band but band 0 of each level, in place
+// (softThreshold.m:15-80: sign(a) max(|a| - tau, 0), low-pass kept)
+template <typename CT>
+__global__ __launch_bounds__(256) void k_shrink_bands(CT* __restrict__ c, size_t band, int nbands, size_t total, float tau)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
+        if ((i / band) % (size_t)nbands == 0) continue;
+        const float a = fl_load(c, i);
+        const float m = fabsf(a) - tau;
+        fl_store(c, i, m > 0.0f ? copysignf(m, a) : 0.0f);
+    }
+}
+
 unsigned grid_for(size_t n)
 {
     size_t b = (n + 255) / 256;
@@ -216,8 +263,11 @@ struct Workspace {
     void* D = nullptr;
     double* fpart = nullptr;
     size_t fpart_n = 0;
+    double* res = nullptr;     // per outer step |Wu - alpha|^2
+    size_t res_n = 0;
     float *Atf = nullptr, *b = nullptr;
-    size_t nb_bands = 0;
+    float* lp = nullptr;       // two low-pass bands of W u (levels > 1)
+    size_t nb_bands = 0;       // coefficient elements of V and D
     int coef_type = FL_F32;
     hipStream_t stream = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
@@ -287,7 +337,8 @@ int ensure_cg(fl_plan_s* pl, Workspace& w, int maxlog, int flags)
 
 void free_ws(Workspace& w)
 {
-    void* ptrs[] = {w.r, w.p, w.q, w.sino, w.u, w.plo, w.slo, w.part, w.st, w.log, w.V, w.D, w.Atf, w.b, w.fpart};
+    void* ptrs[] = {w.r, w.p, w.q, w.sino, w.u, w.plo, w.slo, w.part, w.st, w.log, w.V, w.D, w.Atf, w.b, w.fpart,
+                    w.res, w.lp};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     drop_graph(w);
@@ -409,7 +460,17 @@ int cg_run(fl_plan_s* pl, Workspace& w, const float* b, float* u_caller, float l
             w.gkey_lam = lam;
             w.gkey_flags = flags;
         }
+        // The captured forwards write the plan's brick buffer, and a stream
+        // under capture takes no part in the plan's forward ordering: order
+        // the replays after any forward in flight on another stream, and
+        // later forwards after the replays.
+        std::lock_guard<std::mutex> lk(pl->fwd_mu);
+        if (pl->fwd_done && pl->fwd_pending) HIPCHK(hipStreamWaitEvent(st, pl->fwd_done, 0));
         for (int k = 0; k < iters; k++) HIPCHK(hipGraphLaunch(w.gexec, st));
+        if (pl->fwd_done) {
+            HIPCHK(hipEventRecord(pl->fwd_done, st));
+            pl->fwd_pending = true;
+        }
     } else {
         for (int k = 0; k < iters; k++)
             if ((rc = cg_body(pl, w, u, lam, flags, st))) return rc;
@@ -466,6 +527,34 @@ bool frame_ok(int dim, int kind, int nx, int ny, int nz)
     if (dim != 2 && dim != 3) return false;
     if (kind != FL_FRAME_HAAR && kind != FL_FRAME_LINEAR && kind != FL_FRAME_CUBIC) return false;
     return nx > 0 && ny > 0 && (dim == 2 || nz > 0);
+}
+
+// g += W^T (lam psi'(W x)) over `level` levels, loss partials in part
+// (nbk per level) and their fixed-order sum in part[nbk * level].  work:
+// level blocks of NB bands.  Level L analyses the low-pass band of level
+// L-1, which the analysis of level L-1 leaves in its band 0 (unpenalised);
+// synthesis then replaces each such band 0 by the reconstruction of the level
+// above (FraRecMultiLevel3D.m:14-17), which is the chain rule's backward.
+int grad_framelet_run(const FlFrameDesc& f, int level, const float* x, float lam, float eps, int with_lowpass,
+                      float* g, float* work, double* part, unsigned nbk, hipStream_t st)
+{
+    const size_t n = (size_t)f.nx * f.ny * f.nz;
+    const int nb = fl_framelet_num_bands(f.dim, f.kind);
+    hipError_t e = hipSuccess;
+    for (int L = 1; L <= level && e == hipSuccess; L++) {
+        const float* src = L == 1 ? x : work + (size_t)(L - 2) * nb * n;
+        e = fl_launch_l1grad(f, src, work + (size_t)(L - 1) * nb * n, lam, eps, with_lowpass, L, L < level ? 1 : 0,
+                             part + (size_t)(L - 1) * nbk, st);
+    }
+    for (int L = level; L >= 2 && e == hipSuccess; L--)
+        e = fl_launch_iwt(f, work + (size_t)(L - 1) * nb * n, FL_F32, work + (size_t)(L - 2) * nb * n, FL_F32, L, 1.0f,
+                          0.0f, nullptr, st);
+    if (e == hipSuccess) e = fl_launch_iwt(f, work, FL_F32, g, FL_F32, 1, 1.0f, 1.0f, g, st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, st, part, (int)(nbk * level), part + (size_t)nbk * level);
+        e = hipGetLastError();
+    }
+    return e == hipSuccess ? FL_OK : fl_hip_fail(e, "framelet gradient");
 }
 
 }  // namespace
@@ -580,10 +669,27 @@ extern "C" int fl_cgls(fl_plan pl, const float* d_b, float* d_u, float lam, int 
     return fl_ok();
 }
 
-extern "C" int fl_split_bregman(fl_plan pl, int kind, const float* d_f, float* d_u, float lam, float mu, int outer,
-                                int inner, double cg_tol, int flags, double* resid_log, void* stream)
+// b <- Atf + mu * W^T d for the multi-level coefficients d (level blocks of
+// nbands bands): levels above the first are synthesised in place into the
+// low-pass band of the level below (FraRecMultiLevel3D.m:14-17: C{L-1}{1}
+// <- Rec(C{L})); d is workspace and rebuilt by the next Bregman step.
+static hipError_t synth_rhs(const FlFrameDesc& f, int level, int nbands, void* D, int ct, float mu, float* b,
+                            const float* Atf, hipStream_t st)
 {
-    if (!pl || !d_f || !d_u || outer < 0 || inner < 0 || !(mu > 0))
+    const size_t n = (size_t)f.nx * f.ny * f.nz, esz = ct == FL_F32 ? 4 : 2;
+    char* base = (char*)D;
+    for (int L = level; L >= 2; L--) {
+        const hipError_t e = fl_launch_iwt(f, base + (size_t)(L - 1) * nbands * n * esz, ct,
+                                           base + (size_t)(L - 2) * nbands * n * esz, ct, L, 1.0f, 0.0f, nullptr, st);
+        if (e != hipSuccess) return e;
+    }
+    return fl_launch_iwt(f, D, ct, b, FL_F32, 1, mu, 1.0f, Atf, st);
+}
+
+extern "C" int fl_split_bregman(fl_plan pl, int kind, int level, const float* d_f, float* d_u, float lam, float mu,
+                                int outer, int inner, double cg_tol, int flags, double* resid_log, void* stream)
+{
+    if (!pl || !d_f || !d_u || outer < 0 || inner < 0 || !(mu > 0) || level < 1 || level > 8)
         return fl_fail(FL_ERR_INVALID, "bad split-Bregman arguments");
     if (pl->nt != 1) return fl_fail(FL_ERR_UNSUPPORTED, "split Bregman on one frame");
     const int dim = pl->cone ? 3 : 2;
@@ -592,65 +698,80 @@ extern "C" int fl_split_bregman(fl_plan pl, int kind, const float* d_f, float* d
     const int nbands = fl_framelet_num_bands(dim, kind);
     const size_t n = (size_t)pl->nvox;
     const int ct = lowp_type(flags);
+    const size_t esz = ct == FL_F32 ? 4 : 2;
+    const size_t ncoef = (size_t)level * nbands * n;
     PlanWs* pw = get_ws(pl);
     std::lock_guard<std::mutex> lk(pw->mu);
     Workspace& w = pw->w;
     int rc;
     if ((rc = ensure_cg(pl, w, inner + 1, flags))) return rc;
-    if (!w.V || w.nb_bands != (size_t)nbands || w.coef_type != ct) {
-        for (void* q : {w.V, w.D, (void*)w.Atf, (void*)w.b})
+    if (!w.V || w.nb_bands != ncoef || w.coef_type != ct || (level > 1 && !w.lp)) {
+        for (void* q : {w.V, w.D, (void*)w.Atf, (void*)w.b, (void*)w.lp})
             if (q) (void)hipFree(q);
         w.V = w.D = nullptr;
-        w.Atf = w.b = nullptr;
-        const size_t esz = ct == FL_F32 ? 4 : 2;
-        if ((rc = ws_alloc(&w.V, nbands * n * esz, "v")) || (rc = ws_alloc(&w.D, nbands * n * esz, "d")) ||
-            (rc = ws_alloc((void**)&w.Atf, n * 4, "Atf")) || (rc = ws_alloc((void**)&w.b, n * 4, "b")))
+        w.Atf = w.b = w.lp = nullptr;
+        w.nb_bands = 0;
+        if ((rc = ws_alloc(&w.V, ncoef * esz, "v")) || (rc = ws_alloc(&w.D, ncoef * esz, "d")) ||
+            (rc = ws_alloc((void**)&w.Atf, n * 4, "Atf")) || (rc = ws_alloc((void**)&w.b, n * 4, "b")) ||
+            (level > 1 && (rc = ws_alloc((void**)&w.lp, 2 * n * 4, "low-pass bands"))))
             return rc;
-        w.nb_bands = nbands;
+        w.nb_bands = ncoef;
         w.coef_type = ct;
     }
     const size_t fb = fl_frame_blocks(f);
-    if (w.fpart_n < fb) {
+    if (w.fpart_n < fb * level) {
         if (w.fpart) (void)hipFree(w.fpart);
         w.fpart = nullptr;
         w.fpart_n = 0;
-        if ((rc = ws_alloc((void**)&w.fpart, fb * sizeof(double), "framelet partials"))) return rc;
-        w.fpart_n = fb;
+        if ((rc = ws_alloc((void**)&w.fpart, fb * level * sizeof(double), "framelet partials"))) return rc;
+        w.fpart_n = fb * level;
+    }
+    if (w.res_n < (size_t)(outer > 0 ? outer : 1)) {
+        if (w.res) (void)hipFree(w.res);
+        w.res = nullptr;
+        w.res_n = 0;
+        if ((rc = ws_alloc((void**)&w.res, (size_t)(outer > 0 ? outer : 1) * sizeof(double), "residuals"))) return rc;
+        w.res_n = (size_t)(outer > 0 ? outer : 1);
     }
     hipStream_t caller = (hipStream_t)stream, st = w.stream;
     if ((rc = begin_on(w, caller))) return rc;
     const int kind_adj = (flags & FL_SOLVE_MATCHED) ? FL_ADJ_MATCHED : FL_ADJ_EXACT;
     // v = alpha = W u0 (so alpha - v = 0 and b = Atf), Atf = A^T f
-    hipError_t e = fl_launch_fwt(f, d_u, FL_F32, w.V, ct, 1, st);
-    if (e != hipSuccess) return fl_hip_fail(e, "W u0");
+    for (int L = 1; L <= level; L++) {
+        const void* src = L == 1 ? (const void*)d_u : (const void*)((const char*)w.V + (size_t)(L - 2) * nbands * n * esz);
+        hipError_t e = fl_launch_fwt(f, src, L == 1 ? FL_F32 : ct, (char*)w.V + (size_t)(L - 1) * nbands * n * esz, ct,
+                                     L, st);
+        if (e != hipSuccess) return fl_hip_fail(e, "W u0");
+    }
     if ((rc = fl_adjoint(pl, d_f, FL_F32, w.Atf, FL_F32, kind_adj, st))) return rc;
     HIPCHK(fl_launch_copy(w.Atf, w.b, n, st));
-    std::vector<double> resid(outer > 0 ? outer : 1, 0.0);
-    double* d_res = nullptr;
-    if ((rc = ws_alloc((void**)&d_res, (size_t)(outer > 0 ? outer : 1) * sizeof(double), "residuals"))) return rc;
     for (int k = 0; k < outer; k++) {
         if ((rc = cg_run(pl, w, w.b, d_u, mu, inner, cg_tol, flags))) break;
-        e = fl_launch_bregman(f, d_u, w.V, w.D, ct, lam / mu, w.fpart, st);
+        // per level: analysis of the previous level's low-pass band of W u
+        // (u itself at level 1), shrink, Bregman update
+        hipError_t e = hipSuccess;
+        for (int L = 1; L <= level && e == hipSuccess; L++) {
+            const float* src = L == 1 ? d_u : w.lp + (size_t)((L - 2) & 1) * n;
+            float* lp_out = L < level ? w.lp + (size_t)((L - 1) & 1) * n : nullptr;
+            const size_t off = (size_t)(L - 1) * nbands * n * esz;
+            e = fl_launch_bregman(f, src, (char*)w.V + off, (char*)w.D + off, ct, lam / mu, L, lp_out,
+                                  w.fpart + (size_t)(L - 1) * fb, st);
+        }
         if (e == hipSuccess) {
-            hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, st, w.fpart, (int)fb, d_res + k);
+            hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, st, w.fpart, (int)(fb * level), w.res + k);
             e = hipGetLastError();
         }
-        if (e == hipSuccess) e = fl_launch_iwt(f, w.D, ct, w.b, FL_F32, 1, mu, 1.0f, w.Atf, st);
+        if (e == hipSuccess) e = synth_rhs(f, level, nbands, w.D, ct, mu, w.b, w.Atf, st);
         if (e != hipSuccess) {
             rc = fl_hip_fail(e, "Bregman update");
             break;
         }
     }
     if (!rc && resid_log && outer > 0) {
-        e = hipMemcpyAsync(resid_log, d_res, (size_t)outer * sizeof(double), hipMemcpyDeviceToHost, st);
+        hipError_t e = hipMemcpyAsync(resid_log, w.res, (size_t)outer * sizeof(double), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) rc = fl_hip_fail(e, "residual log");
     }
-    if (!rc) {
-        e = hipStreamSynchronize(st);
-        if (e != hipSuccess) rc = fl_hip_fail(e, "split Bregman");
-    }
-    (void)hipFree(d_res);
     if (rc) return rc;
     if ((rc = end_on(w, caller))) return rc;
     return fl_ok();
@@ -665,7 +786,11 @@ extern "C" int fl_grad_data(fl_plan pl, const float* d_x, const float* d_b, floa
     Workspace& w = pw->w;
     int rc;
     if ((rc = ensure_cg(pl, w, 1, 0))) return rc;
-    hipStream_t st = (hipStream_t)stream;
+    // the residual sinogram and the partials are plan workspace, shared with
+    // the solvers: the gradient runs on the workspace stream, ordered after
+    // the caller's stream and before the caller's later work
+    hipStream_t caller = (hipStream_t)stream, st = w.stream;
+    if ((rc = begin_on(w, caller))) return rc;
     const size_t ns = (size_t)pl->ndet * pl->nv;
     if ((rc = fl_forward(pl, d_x, FL_F32, w.sino, FL_F32, st))) return rc;
     const unsigned g = grid_for(ns);
@@ -679,27 +804,110 @@ extern "C" int fl_grad_data(fl_plan pl, const float* d_x, const float* d_b, floa
         HIPCHK(hipMemcpyAsync(loss, out, sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
     }
+    if ((rc = end_on(w, caller))) return rc;
     return fl_ok();
 }
 
-extern "C" int fl_grad_framelet(int dim, int kind, int nx, int ny, int nz, const float* d_x, float lam, float eps,
-                                int with_lowpass, float* d_g, float* d_work, double* loss, void* stream)
+extern "C" int fl_grad_framelet(int dim, int kind, int level, int nx, int ny, int nz, const float* d_x, float lam,
+                                float eps, int with_lowpass, float* d_g, float* d_work, double* loss, void* stream)
 {
-    if (!frame_ok(dim, kind, nx, ny, nz) || !d_x || !d_g || !d_work || !(eps > 0))
+    if (!frame_ok(dim, kind, nx, ny, nz) || !d_x || !d_g || !d_work || !(eps > 0) || level < 1 || level > 8)
         return fl_fail(FL_ERR_INVALID, "bad framelet gradient arguments");
     FlFrameDesc f = {dim, kind, nx, ny, dim == 3 ? nz : 1};
     const unsigned nbk = fl_frame_blocks(f);
-    double* part = device_scratch_doubles((size_t)nbk + 1);
+    double* part = device_scratch_doubles((size_t)nbk * level + 1);
     if (!part) return fl_fail(FL_ERR_ALLOC, "framelet partials");
-    hipStream_t st = (hipStream_t)stream;
-    hipError_t e = fl_launch_l1grad(f, d_x, d_work, lam, eps, with_lowpass, part, st);
-    if (e == hipSuccess) e = fl_launch_iwt(f, d_work, FL_F32, d_g, FL_F32, 1, 1.0f, 1.0f, d_g, st);
-    if (e != hipSuccess) return fl_hip_fail(e, "framelet gradient");
+    const int rc = grad_framelet_run(f, level, d_x, lam, eps, with_lowpass, d_g, d_work, part, nbk, (hipStream_t)stream);
+    if (rc) return rc;
     if (loss) {
-        hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, st, part, (int)nbk, part + nbk);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(loss, part + nbk, sizeof(double), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipMemcpyAsync(loss, part + (size_t)nbk * level, sizeof(double), hipMemcpyDeviceToHost,
+                              (hipStream_t)stream));
+        HIPCHK(hipStreamSynchronize((hipStream_t)stream));
     }
+    return fl_ok();
+}
+
+// ------------------------------------- CGLS over caller-supplied operators --
+
+namespace {
+// FP64 partials of the generic vector steps: summed in a fixed order on the
+// device, the sum copied to the host (the step synchronises the stream)
+int sum_to_host(const double* part, unsigned np, double* out, hipStream_t st)
+{
+    double* d_out = const_cast<double*>(part) + NPART;
+    hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, st, part, (int)np, d_out);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, d_out, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return FL_OK;
+}
+}  // namespace
+
+extern "C" int fl_vec_cgls_init(long long n, const float* d_q, const float* d_u, const float* d_b, float lam,
+                                float* d_r, float* d_p, double* gamma, void* stream)
+{
+    if (n <= 0 || !d_q || !d_u || !d_b || !d_r || !d_p || !gamma)
+        return fl_fail(FL_ERR_INVALID, "bad CGLS vector arguments");
+    double* part = device_scratch_doubles(NPART + 1);
+    if (!part) return fl_fail(FL_ERR_ALLOC, "CGLS partials");
+    hipStream_t st = (hipStream_t)stream;
+    const unsigned g = grid_for((size_t)n);
+    hipLaunchKernelGGL((k_cg_init<float>), dim3(g), dim3(256), 0, st, d_q, d_u, d_b, lam, d_r, d_p, (float*)nullptr,
+                       (size_t)n, part);
+    const int rc = sum_to_host(part, g, gamma, st);
+    return rc ? rc : fl_ok();
+}
+
+extern "C" int fl_vec_cgls_pq(long long n, float* d_q, const float* d_p, float lam, double* pq, void* stream)
+{
+    if (n <= 0 || !d_q || !d_p || !pq) return fl_fail(FL_ERR_INVALID, "bad CGLS vector arguments");
+    double* part = device_scratch_doubles(NPART + 1);
+    if (!part) return fl_fail(FL_ERR_ALLOC, "CGLS partials");
+    hipStream_t st = (hipStream_t)stream;
+    const unsigned g = grid_for((size_t)n);
+    hipLaunchKernelGGL(k_gv_pq, dim3(g), dim3(256), 0, st, d_q, d_p, lam, (size_t)n, part);
+    const int rc = sum_to_host(part, g, pq, st);
+    return rc ? rc : fl_ok();
+}
+
+extern "C" int fl_vec_cgls_update(long long n, float* d_u, float* d_r, const float* d_p, const float* d_q,
+                                  double alpha, double* gamma, void* stream)
+{
+    if (n <= 0 || !d_u || !d_r || !d_p || !d_q || !gamma) return fl_fail(FL_ERR_INVALID, "bad CGLS vector arguments");
+    double* part = device_scratch_doubles(NPART + 1);
+    if (!part) return fl_fail(FL_ERR_ALLOC, "CGLS partials");
+    hipStream_t st = (hipStream_t)stream;
+    const unsigned g = grid_for((size_t)n);
+    hipLaunchKernelGGL(k_gv_update, dim3(g), dim3(256), 0, st, d_u, d_r, d_p, d_q, (float)alpha, (size_t)n, part);
+    const int rc = sum_to_host(part, g, gamma, st);
+    return rc ? rc : fl_ok();
+}
+
+extern "C" int fl_vec_cgls_dir(long long n, float* d_p, const float* d_r, double beta, void* stream)
+{
+    if (n <= 0 || !d_p || !d_r) return fl_fail(FL_ERR_INVALID, "bad CGLS vector arguments");
+    hipLaunchKernelGGL(k_gv_dir, dim3(grid_for((size_t)n)), dim3(256), 0, (hipStream_t)stream, d_p, d_r, (float)beta,
+                       (size_t)n);
+    HIPCHK(hipGetLastError());
+    return fl_ok();
+}
+
+extern "C" int fl_framelet_shrink(int nbands, int level, long long band_elems, void* d_c, int c_dtype, float tau,
+                                  void* stream)
+{
+    if (nbands <= 0 || level < 1 || band_elems <= 0 || !d_c || !(tau >= 0))
+        return fl_fail(FL_ERR_INVALID, "bad soft-threshold arguments");
+    const size_t band = (size_t)band_elems, total = band * nbands * level;
+    const unsigned g = grid_for(total);
+    hipStream_t st = (hipStream_t)stream;
+    if (c_dtype == FL_F32)
+        hipLaunchKernelGGL(k_shrink_bands<float>, dim3(g), dim3(256), 0, st, (float*)d_c, band, nbands, total, tau);
+    else if (c_dtype == FL_BF16)
+        hipLaunchKernelGGL(k_shrink_bands<bf16_t>, dim3(g), dim3(256), 0, st, (bf16_t*)d_c, band, nbands, total, tau);
+    else if (c_dtype == FL_F16)
+        hipLaunchKernelGGL(k_shrink_bands<f16_t>, dim3(g), dim3(256), 0, st, (f16_t*)d_c, band, nbands, total, tau);
+    else
+        return fl_fail(FL_ERR_INVALID, "bad coefficient dtype");
+    HIPCHK(hipGetLastError());
     return fl_ok();
 }

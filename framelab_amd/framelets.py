@@ -110,9 +110,13 @@ class FrameletExpansion:
     __mul__ = __rmul__
 
     def softThreshold(self, tau: float) -> "FrameletExpansion":
-        """sign(a)*max(|a|-tau,0) on every band except each level's low-pass."""
+        """sign(a)*max(|a|-tau,0) on every band except each level's low-pass
+        (native kernel, in place; also returned)."""
         a = self.frameletArray
-        lp = a[:, 0].clone()
-        a.copy_(torch.sign(a) * torch.clamp(a.abs() - tau, min=0))
-        a[:, 0] = lp
+        if not a.is_contiguous():
+            raise ValueError("frameletArray must be contiguous")
+        level, nb = a.shape[0], a.shape[1]
+        band = a[0, 0].numel()
+        _lib.check(_lib.lib().fl_framelet_shrink(nb, level, band, C.c_void_p(a.data_ptr()), _DT[a.dtype],
+                                                 float(tau), _stream()))
         return self

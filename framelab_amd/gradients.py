@@ -36,15 +36,14 @@ def data_gradient(P: Projector, x: torch.Tensor, b: torch.Tensor, exact_adjoint:
 
 def framelet_gradient(W: FrameletSystem, x: torch.Tensor, lam: float, eps: float = 1e-6,
                       with_lowpass: bool = False, accumulate_into: torch.Tensor | None = None):
-    """(g, loss) with g = lam * W^T psi'(W x) (+ accumulate_into), loss = lam * sum psi(W x)."""
-    if W.level != 1:
-        raise NotImplementedError("the smoothed-l1 gradient is implemented for one level")
+    """(g, loss) with g = lam * W^T psi'(W x) (+ accumulate_into), loss = lam * sum psi(W x) over
+    every band of every level but the low-pass ones (the top level's too unless with_lowpass)."""
     x = x.float().contiguous()
     g = accumulate_into if accumulate_into is not None else torch.zeros_like(x)
-    work = torch.empty((W.nbands,) + tuple(x.shape), dtype=torch.float32, device=x.device)
+    work = torch.empty((W.level * W.nbands,) + tuple(x.shape), dtype=torch.float32, device=x.device)
     nx, ny, nz = W._sizes(x.shape)
     loss = C.c_double(0.0)
-    _lib.check(_lib.lib().fl_grad_framelet(W.dim, W.kind, nx, ny, nz, C.c_void_p(x.data_ptr()), float(lam),
+    _lib.check(_lib.lib().fl_grad_framelet(W.dim, W.kind, W.level, nx, ny, nz, C.c_void_p(x.data_ptr()), float(lam),
                                            float(eps), int(with_lowpass), C.c_void_p(g.data_ptr()),
                                            C.c_void_p(work.data_ptr()), C.byref(loss), _stream()))
     return g, loss.value

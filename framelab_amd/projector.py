@@ -156,12 +156,40 @@ class Projector:
                                          _stream()))
         return out
 
+    def forward_f64(self, x: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """y = A x in FP64 (the Siddon traversal rule in double): the accuracy reference."""
+        if x.dtype != torch.float64 or not x.is_cuda or not x.is_contiguous() or x.numel() != self.nvox * self.nt:
+            raise ValueError("x must be a contiguous float64 CUDA tensor of the volume's size")
+        if out is None:
+            out = torch.empty(self.sino_shape, dtype=torch.float64, device=x.device)
+        if out.dtype != torch.float64 or not out.is_contiguous() or out.numel() != self.ndet * self.nv:
+            raise ValueError("out must be a contiguous float64 tensor of the sinogram's size")
+        _lib.check(_lib.lib().fl_forward_f64(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(out.data_ptr()),
+                                             _stream()))
+        return out
+
+    def adjoint_f64(self, y: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """x = A^T y in FP64 (voxel-driven transpose of forward_f64)."""
+        if y.dtype != torch.float64 or not y.is_cuda or not y.is_contiguous() or y.numel() != self.ndet * self.nv:
+            raise ValueError("y must be a contiguous float64 CUDA tensor of the sinogram's size")
+        if out is None:
+            out = torch.empty(self.vol_shape, dtype=torch.float64, device=y.device)
+        if out.dtype != torch.float64 or not out.is_contiguous() or out.numel() != self.nvox * self.nt:
+            raise ValueError("out must be a contiguous float64 tensor of the volume's size")
+        _lib.check(_lib.lib().fl_adjoint_f64(self._h, C.c_void_p(y.data_ptr()), C.c_void_p(out.data_ptr()),
+                                             _stream()))
+        return out
+
     def forward_norm(self, y: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """yNorm = y / (|d| * scale) per ray (cone, FP32): the reference gateway's second output."""
         self._check_tensor(y, self.ndet * self.nv, "y")
+        if y.dtype != torch.float32:
+            raise TypeError(f"forward_norm reads FP32 sinograms, got {y.dtype}")
         if out is None:
             out = torch.empty_like(y)
         self._check_tensor(out, self.ndet * self.nv, "out")
+        if out.dtype != torch.float32:
+            raise TypeError(f"forward_norm writes FP32, got out of {out.dtype}")
         _lib.check(_lib.lib().fl_forward_norm(self._h, C.c_void_p(y.data_ptr()), C.c_void_p(out.data_ptr()),
                                               _stream()))
         return out

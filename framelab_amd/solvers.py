@@ -29,6 +29,8 @@ def _projector_of(A):
     if isinstance(A, Projector):
         return A
     owner = getattr(A, "__self__", None)
+    if isinstance(owner, Projector):        # P.forward
+        return owner
     return getattr(owner, "_proj", None) if owner is not None else None
 
 
@@ -63,15 +65,29 @@ def cgls(P: Projector, b: torch.Tensor, u0: torch.Tensor, lam: float = 0.0, iter
     return u, (np.array(glog[:]) if log else None), done.value
 
 
+def cgls_f64(P: Projector, b: torch.Tensor, u0: torch.Tensor, lam: float = 0.0, iters: int = 20,
+             tol: float = 0.0):
+    """CGLS with FP64 vectors over the FP64 projector pair (the accuracy
+    reference of the FP32 and mixed-precision solves).  Returns (u, gamma_log, iters_done)."""
+    u = u0.detach().to(torch.float64).contiguous().clone()
+    b = b.to(torch.float64).contiguous()
+    glog = (C.c_double * (iters + 1))()
+    done = C.c_int(0)
+    _lib.check(_lib.lib().fl_cgls_f64(P.handle, C.c_void_p(b.data_ptr()), C.c_void_p(u.data_ptr()), float(lam),
+                                      int(iters), float(tol), glog, C.byref(done), _stream()))
+    return u, np.array(glog[:]), done.value
+
+
 def split_bregman(P: Projector, kind: str, f: torch.Tensor, u0: torch.Tensor, lam: float, mu: float,
                   outer: int, inner: int, cg_tol: float = 0.0, graph: bool = False, matched: bool = False,
-                  storage=torch.float32):
-    """Device-resident split Bregman.  Returns (u, constraint residual per outer step)."""
+                  storage=torch.float32, level: int = 1):
+    """Device-resident split Bregman with a `level`-level frame.  Returns (u,
+    constraint residual per outer step)."""
     from .framelets import _KIND
     u = u0.detach().float().contiguous().clone()
     f = f.float().contiguous()
     res = (C.c_double * max(outer, 1))()
-    _lib.check(_lib.lib().fl_split_bregman(P.handle, _KIND[kind], C.c_void_p(f.data_ptr()),
+    _lib.check(_lib.lib().fl_split_bregman(P.handle, _KIND[kind], int(level), C.c_void_p(f.data_ptr()),
                                            C.c_void_p(u.data_ptr()), float(lam), float(mu), int(outer),
                                            int(inner), float(cg_tol), _flags(graph, matched, storage), res,
                                            _stream()))
@@ -100,25 +116,38 @@ class CGLS:
         return ObjectData(self.u0.dim, u, self.u0.L) if isinstance(self.u0, ObjectData) else u
 
     def _generic(self, b, u0):
-        """solveCGLS.m:7-30 over arbitrary callables (FP64 dots)."""
-        A, At, lam = self.A, self.At, self.lam
-        Aloc = lambda v: _data(At(A(v))) + lam * _data(v)
-        u = u0.clone()
-        r = Aloc(u) - b
-        p = -r
-        dot = lambda a, c: float(torch.dot(a.reshape(-1).double(), c.reshape(-1).double()))
-        gold = dot(r, r)
-        log = [gold]
+        """solveCGLS.m:7-30 over arbitrary callables: the operators are the
+        caller's, the vector steps and FP64 dots are the native ones."""
+        A, At, lam = self.A, self.At, float(self.lam)
+        L = _lib.lib()
+        u = u0.detach().float().contiguous().clone()
+        b = b.float().contiguous()
+        if not (u.is_cuda and b.is_cuda):
+            raise TypeError("CGLS vectors must be CUDA (HIP) tensors")
+        n = u.numel()
+        ptr = lambda t: C.c_void_p(t.data_ptr())
+
+        def normal(v):
+            return _data(At(A(v))).float().contiguous()
+
+        r, p = torch.empty_like(u), torch.empty_like(u)
+        gold = C.c_double(0.0)
+        _lib.check(L.fl_vec_cgls_init(n, ptr(normal(u)), ptr(u), ptr(b), lam, ptr(r), ptr(p), C.byref(gold),
+                                      _stream()))
+        log = [gold.value]
         for _ in range(self.globalIter):
-            q = Aloc(p)
-            alpha = gold / dot(p, q)
-            u = u + alpha * p
-            r = r + alpha * q
-            gnew = dot(r, r)
-            log.append(gnew)
-            p = -r + (gnew / gold) * p
+            q = normal(p)
+            pq = C.c_double(0.0)
+            _lib.check(L.fl_vec_cgls_pq(n, ptr(q), ptr(p), lam, C.byref(pq), _stream()))
+            if pq.value == 0.0:
+                break
+            gnew = C.c_double(0.0)
+            _lib.check(L.fl_vec_cgls_update(n, ptr(u), ptr(r), ptr(p), ptr(q), gold.value / pq.value,
+                                            C.byref(gnew), _stream()))
+            log.append(gnew.value)
+            _lib.check(L.fl_vec_cgls_dir(n, ptr(p), ptr(r), gnew.value / gold.value, _stream()))
             gold = gnew
-            if gold < self.convergenceTol:
+            if gold.value < self.convergenceTol:
                 break
         self.gamma_log = np.array(log)
         return u
@@ -139,11 +168,11 @@ class SplitBregman:
         P = _projector_of(self.A)
         system = getattr(self.W, "__self__", None)
         f, u0 = _data(self.f), _data(self.u0)
-        if P is None or not isinstance(system, FrameletSystem) or system.level != 1:
-            raise NotImplementedError("split Bregman runs on this package's projectors with a one-level "
+        if P is None or not isinstance(system, FrameletSystem):
+            raise NotImplementedError("split Bregman runs on this package's projectors with a "
                                       "FrameletSystem (W=system.forwardTransform)")
         matched = getattr(getattr(self.At, "__self__", None), "adjoint_kind", 0) == 1
         u, self.residuals = split_bregman(P, system.type, f, u0, self.lam, self.mu, self.globalIter,
                                           self.cgIter, self.cgTol, graph=self.graph, matched=matched,
-                                          storage=self.storage)
+                                          storage=self.storage, level=system.level)
         return ObjectData(self.u0.dim, u, self.u0.L) if isinstance(self.u0, ObjectData) else u

@@ -198,12 +198,39 @@ int fl_cgls(fl_plan plan, const float *d_b, float *d_u, float lam, int iters, do
 /* Split Bregman with the framelet analysis model.  Replaces
  * SplitBregman.solveSplitBregman (matlab/classes/+Optimizers/@SplitBregman/
  * solveSplitBregman.m:1-44): outer iterations of an `inner`-step CGLS with
- * lam = mu, shrink(v + Wu, lam/mu) (low-pass kept), v <- v + (Wu - alpha).
+ * lam = mu, shrink(v + Wu, lam/mu) (every level's low-pass band kept),
+ * v <- v + (Wu - alpha), b = A^T f + mu W^T (alpha - v), with the `level`-
+ * level frame of FrameletSystem(dim, kind, level) (FrameletSystem.m:18-27).
  * d_f: sinogram; d_u: u0 in, u out.  resid_log (outer doubles, optional):
- * |Wu - alpha|^2 after each outer step. */
-int fl_split_bregman(fl_plan plan, int frame_kind, const float *d_f, float *d_u, float lam,
-                     float mu, int outer, int inner, double cg_tol, int flags, double *resid_log,
-                     void *stream);
+ * |Wu - alpha|^2 over all levels after each outer step (requesting it
+ * synchronises). */
+int fl_split_bregman(fl_plan plan, int frame_kind, int level, const float *d_f, float *d_u,
+                     float lam, float mu, int outer, int inner, double cg_tol, int flags,
+                     double *resid_log, void *stream);
+
+/* CGLS over caller-supplied operators (the class mirror's CGLS.solveCGLS
+ * with arbitrary A / At): the vector steps of solveCGLS.m:7-30 on FP32
+ * device vectors with FP64 dot products returned to the host (each call
+ * with a host output synchronises the stream).  The caller forms
+ * q = At(A(.)) between the calls.
+ *   init:   r = q + lam u - b, p = -r, *gamma = r.r
+ *   pq:     q = q + lam p, *pq = p.q
+ *   update: u = u + alpha p, r = r + alpha q, *gamma = r.r
+ *   dir:    p = -r + beta p */
+int fl_vec_cgls_init(long long n, const float *d_q, const float *d_u, const float *d_b, float lam,
+                     float *d_r, float *d_p, double *gamma, void *stream);
+int fl_vec_cgls_pq(long long n, float *d_q, const float *d_p, float lam, double *pq, void *stream);
+int fl_vec_cgls_update(long long n, float *d_u, float *d_r, const float *d_p, const float *d_q,
+                       double alpha, double *gamma, void *stream);
+int fl_vec_cgls_dir(long long n, float *d_p, const float *d_r, double beta, void *stream);
+
+/* Soft threshold in place of framelet coefficients (`level` blocks of
+ * `nbands` bands of band_elems elements): sign(a) max(|a| - tau, 0) on every
+ * band but each level's low-pass band 0.  Replaces
+ * FrameletExpansion.softThreshold (matlab/classes/+DataTypes/
+ * @FrameletExpansion/softThreshold.m:1-80, shrinkFramelet2D/3D). */
+int fl_framelet_shrink(int nbands, int level, long long band_elems, void *d_c, int c_dtype, float tau,
+                       void *stream);
 
 /* ----------------------------------------------------------- gradients */
 
@@ -214,14 +241,17 @@ int fl_split_bregman(fl_plan plan, int frame_kind, const float *d_f, float *d_u,
 int fl_grad_data(fl_plan plan, const float *d_x, const float *d_b, float *d_g, double *loss,
                  int flags, void *stream);
 
-/* g += W^T (lam * psi'(W x)) with psi(t) = sqrt(t^2 + eps) (smoothed l1),
- * loss = lam * sum psi (optional); one level; the low-pass band is left
- * unpenalised unless with_lowpass.  d_work: NB*nz*ny*nx floats.  d_g is
- * accumulated into (pass zeros for the framelet term alone).  No reference
- * counterpart: the backward of the analysis transform. */
-int fl_grad_framelet(int dim, int kind, int nx, int ny, int nz, const float *d_x, float lam,
-                     float eps, int with_lowpass, float *d_g, float *d_work, double *loss,
-                     void *stream);
+/* g += W^T (lam * psi'(W x)) with psi(t) = sqrt(t^2 + eps) (smoothed l1)
+ * over the bands of a `level`-level frame, loss = lam * sum psi (optional,
+ * host; requesting it synchronises).  Every level's low-pass band is left
+ * unpenalised (it is not a coefficient of the multi-level frame: the
+ * synthesis replaces it) except the top level's when with_lowpass.  d_work:
+ * level*NB*nz*ny*nx floats.  d_g is accumulated into (pass zeros for the
+ * framelet term alone).  No reference counterpart: the backward of the
+ * analysis transform. */
+int fl_grad_framelet(int dim, int kind, int level, int nx, int ny, int nz, const float *d_x,
+                     float lam, float eps, int with_lowpass, float *d_g, float *d_work,
+                     double *loss, void *stream);
 
 /* ------------------------------------------------------------- batches
  * Many independent 2D fan-beam problems of uneven sizes (one frame each),
@@ -283,24 +313,24 @@ int fl_john_normal(fl_john john, const void *d_g, int g_dtype, double *d_work, d
  * cone plans, FP32. */
 int fl_forward_norm(fl_plan plan, const float *d_y, float *d_ynorm, void *stream);
 
+/* ------------------------------------------------- FP64 reference pair
+ * The Siddon forward in double (the reference's traversal rule:
+ * matlab/mex/src/Ax_cone_mf_cpu_siddon.cpp:35-131 with every float
+ * operation in double; view trig cos/sin of the angle in double) and its
+ * voxel-driven transpose, on FP64 device buffers laid out as fl_forward /
+ * fl_adjoint's; and CGLS (solveCGLS.m:7-30) with FP64 vectors over that
+ * pair.  The accuracy reference for the FP32 / BF16 / FP16 paths at sizes
+ * no CPU oracle reaches.  No reference counterpart (the reference is FP32).
+ * fl_cgls_f64 allocates its workspace per call and returns after the solve
+ * (it synchronises the stream). */
+int fl_forward_f64(fl_plan plan, const double *d_x, double *d_y, void *stream);
+int fl_adjoint_f64(fl_plan plan, const double *d_y, double *d_x, void *stream);
+int fl_cgls_f64(fl_plan plan, const double *d_b, double *d_u, double lam, int iters, double tol,
+                double *gamma_log, int *iters_done, void *stream);
+
 /* Traversal log of selected rays (testing): rays = (view, ia, ib) triples
  * (host), cap entries per ray; J/S/cnt host outputs of nrays*cap / nrays. */
 int fl_siddon_trace(fl_plan plan, int nrays, const int *rays, int cap, int *J, float *S, int *cnt);
-
-/* Measurement helper (no reference counterpart): eight independent chains
- * of `iters` steps per thread, of FP32 FMAs (kind 0) or of 32-bit integer
- * shift/add/xor (kind 1), one store per thread to d_out (n_threads floats, a
- * positive multiple of 256).  Their per-dispatch VALU instruction rates
- * under the profiler are the issue peaks the projectors' rates are compared
- * with. */
-int fl_calibrate_valu(float *d_out, int n_threads, int iters, int kind, void *stream);
-
-/* Self-check (no reference counterpart): the exact adjoint's chord length
- * uses its own square root (the correctly rounded expansion without the
- * rescaling of tiny inputs, which take sqrtf).  Compares it with sqrtf on
- * every non-negative finite float and leaves the number of bit mismatches
- * in the device counter *d_bad (zeroed by the call). */
-int fl_check_chord_sqrt(unsigned long long *d_bad, void *stream);
 
 #ifdef __cplusplus
 }

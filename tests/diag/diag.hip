@@ -1,5 +1,7 @@
-// calib.hip -- VALU calibration kernel for counter-based rooflines, and an
-// exhaustive self-check of the exact adjoint's square root.
+// diag.hip -- measurement and self-check kernels (libfl_diag.so; test and
+// profiling infrastructure, not part of libframelab_hip.so): VALU issue-peak
+// calibration for counter-based rooflines, and an exhaustive check of the
+// exact adjoint's square root against sqrtf.
 //
 // The projectors are bound by vector-ALU issue, not HBM.  To state what
 // fraction of the VALU peak they reach, the profiler's per-dispatch VALU
@@ -9,9 +11,7 @@
 // each issues VALU instructions of its kind at the rate the hardware allows.
 #include <hip/hip_runtime.h>
 
-#include "../../include/framelab_hip.h"
-#include "fl_capi_internal.h"
-#include "fl_chord.h"
+#include "../../framelab_amd/csrc/fl_chord.h"
 
 namespace {
 
@@ -60,26 +60,23 @@ __global__ void k_zero_u64(unsigned long long* __restrict__ v) { *v = 0; }
 
 }  // namespace
 
+// Status: 0 ok, -1 invalid argument, -2 HIP launch error.
 extern "C" int fl_check_chord_sqrt(unsigned long long* d_bad, void* stream)
 {
-    if (!d_bad) return fl_fail(FL_ERR_INVALID, "null counter");
+    if (!d_bad) return -1;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, (hipStream_t)stream, d_bad);
     hipLaunchKernelGGL(k_check_sqrt, dim3(65536), dim3(256), 0, (hipStream_t)stream, d_bad);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return fl_hip_fail(e, "fl_check_chord_sqrt");
-    return fl_ok();
+    return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 extern "C" int fl_calibrate_valu(float* d_out, int n_threads, int iters, int kind, void* stream)
 {
     if (!d_out || n_threads <= 0 || (n_threads % 256) != 0 || iters <= 0 || (kind != 0 && kind != 1))
-        return fl_fail(FL_ERR_INVALID, "n_threads must be a positive multiple of 256, iters positive, kind 0 or 1");
+        return -1;   // n_threads: a positive multiple of 256; iters positive; kind 0 or 1
     const dim3 grid(n_threads / 256);
     if (kind == 0)
         hipLaunchKernelGGL(k_valu_peak<0>, grid, dim3(256), 0, (hipStream_t)stream, d_out, iters, 0.999f, 1e-3f);
     else
         hipLaunchKernelGGL(k_valu_peak<1>, grid, dim3(256), 0, (hipStream_t)stream, d_out, iters, 0.999f, 1e-3f);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return fl_hip_fail(e, "fl_calibrate_valu");
-    return fl_ok();
+    return hipGetLastError() == hipSuccess ? 0 : -2;
 }

@@ -516,9 +516,10 @@ def test_chord_sqrt_exhaustive(gpu):
     without the rescaling of inputs below 2^-96, which take sqrtf) returns
     sqrtf's bits for every one of the 2^31 - 2^23 non-negative finite
     floats."""
-    from framelab_amd import _lib
+    from diag import diag
     _say("case: chord square root, all non-negative finite floats")
     bad = torch.zeros(1, dtype=torch.int64, device=gpu)
-    _lib.check(_lib.lib().fl_check_chord_sqrt(C.c_void_p(bad.data_ptr()),
-                                              C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    diag.check(diag.lib().fl_check_chord_sqrt(C.c_void_p(bad.data_ptr()),
+                                              C.c_void_p(torch.cuda.current_stream().cuda_stream)),
+               "fl_check_chord_sqrt")
     assert int(bad.item()) == 0

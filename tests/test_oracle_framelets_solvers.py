@@ -106,3 +106,57 @@ def test_split_bregman_denoising_fixed_point(oracle):
     u8 = oracle.split_bregman(I, I, W, Wt, lam, mu, f, f.copy(), 8, 20, shrink=shrink)
     assert obj(u8) < obj(f)
     assert obj(u8) <= obj(u1) + 1e-9
+
+
+def _highpass_matrix(oracle, kind, shape, level):
+    """The analysis rows the split-Bregman objective penalises, as a dense
+    matrix: every band but the low-pass one of every level (level L analyses
+    the low-pass band of level L-1), built column by column from fradec2d."""
+    D, _ = oracle.framelet_filters(kind)
+    n = shape[0] * shape[1]
+    cols = []
+    for k in range(n):
+        e = np.zeros(n)
+        e[k] = 1.0
+        C = oracle.fradec2d(e.reshape(shape), D, level)
+        rows = []
+        for c in C:
+            flat = c.reshape(-1, n)
+            rows.append(flat[1:])                 # band (0, 0) is the low-pass
+        cols.append(np.concatenate(rows).ravel())
+    return np.stack(cols, axis=1)
+
+
+@pytest.mark.parametrize("level", [1, 2])
+def test_split_bregman_reaches_the_minimiser(oracle, level):
+    """With A = I, split Bregman (solveSplitBregman.m:16-41, class form) must
+    converge to the minimiser of 0.5|u - f|^2 + lam |B u|_1, B = the
+    high-pass analysis rows of the `level`-level linear frame (its fixed
+    point: alpha = shrink(v + W u) with W u = alpha on those rows).  The
+    minimiser is computed independently from the dual problem
+    min_{|p|_inf <= lam} 0.5|f - B^T p|^2, a box-constrained least-squares
+    problem (scipy's lsq_linear, bounded-variable LS), u* = f - B^T p*, and
+    checked for optimality against random perturbations.  A dropped
+    Bregman term, a wrong sign in v's update or a shrunk low-pass band moves
+    the iterate off u*."""
+    from scipy.optimize import lsq_linear
+    rng = np.random.default_rng(6)
+    shape = (8, 8)
+    f = np.zeros(shape)
+    f[2:6, 1:5] = 1.0
+    f += 0.2 * rng.standard_normal(shape)
+    lam, mu = 0.08, 4.0
+    B = _highpass_matrix(oracle, "linear", shape, level)
+    sol = lsq_linear(B.T, f.ravel(), bounds=(-lam, lam), method="bvls", tol=1e-15, max_iter=10000)
+    ustar = f.ravel() - B.T @ sol.x
+    obj = lambda v: 0.5 * np.sum((v - f.ravel()) ** 2) + lam * np.abs(B @ v).sum()
+    for _ in range(20):
+        assert obj(ustar) <= obj(ustar + 1e-3 * rng.standard_normal(ustar.size)) + 1e-12
+    D, R = oracle.framelet_filters("linear")
+    W = lambda u: np.stack(oracle.fradec2d(u.reshape(shape), D, level))
+    Wt = lambda c: oracle.frarec2d(list(c), R)
+    shrink = lambda c, t: np.stack(oracle.soft_threshold(list(c), t, 2))
+    I = lambda v: v
+    u = oracle.split_bregman(I, I, W, Wt, lam, mu, f, f.copy(), 3000, 2, shrink=shrink)
+    err = np.linalg.norm(u.ravel() - ustar) / np.linalg.norm(ustar)
+    assert err < 1e-9, err

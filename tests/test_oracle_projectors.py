@@ -285,3 +285,63 @@ def test_trace_matches_projection(oracle):
         for j, s in zip(J, S):
             acc = np.float32(acc + np.float32(s * X.ravel()[j]))
         assert np.float32(acc * p.scale) == y[iv, ib, ia]
+
+
+def _cone_column_sums_fp64(p):
+    """A^T 1 by brute force: for every ray (view, row, column) the FP64 chord
+    of the line source -> detector cell through every voxel box (slab
+    clipping, vectorised over the voxels), summed, times scale."""
+    nx, ny, nz, dz = p.nx, p.ny, p.nz, float(p.dz)
+    xc = np.arange(nx) + 0.5 - nx // 2
+    yc = np.arange(ny) + 0.5 - ny // 2
+    zc = (np.arange(nz) + 0.5 - nz // 2) * dz
+    Z, Y, X = np.meshgrid(zc, yc, xc, indexing="ij")
+    centre = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    half = np.array([0.5, 0.5, dz / 2])
+    out = np.zeros(centre.shape[0])
+    for iv in range(p.nv):
+        c, s = float(p.cos_phi[iv]), float(p.sin_phi[iv])
+        sz = float(p.sd_z[iv])
+        src = np.array([-float(p.SO) * c, -float(p.SO) * s, sz])
+        for ib in range(p.nb):
+            for ia in range(p.na):
+                yd = float(p.y_det[ia])
+                det = np.array([c * float(p.OD) - s * yd, s * float(p.OD) + c * yd, float(p.z_det[ib]) + sz])
+                d = det - src
+                t0 = np.full(centre.shape[0], -np.inf)
+                t1 = np.full(centre.shape[0], np.inf)
+                inside = np.ones(centre.shape[0], bool)
+                for k in range(3):
+                    lo, hi = centre[:, k] - half[k] - src[k], centre[:, k] + half[k] - src[k]
+                    if d[k] == 0.0:
+                        inside &= (lo <= 0) & (hi >= 0)
+                        continue
+                    a, b = lo / d[k], hi / d[k]
+                    t0 = np.maximum(t0, np.minimum(a, b))
+                    t1 = np.minimum(t1, np.maximum(a, b))
+                out += np.where(inside, np.maximum(t1 - t0, 0.0), 0.0) * np.linalg.norm(d)
+    return out * float(p.scale)
+
+
+@pytest.mark.parametrize("kind", ["dz", "helix"])
+def test_cone_adjoint_column_sums(oracle, kind):
+    """A^T 1 of the exact cone adjoint (GPU semantics, the restatement of
+    Atx_cone_mf_gpu_new.cu:41-84 that _ref cannot check off dz = 1, sd_z = 0)
+    equals the brute-force FP64 sum of voxel chords over all rays: voxels
+    0.8 high (window half-height d = sqrt((1+dz^2)/2), box side dz), and a
+    helical orbit (per-view source height sd_z in the row window and the
+    chord).  Pins the window (a cell dropped from it loses that chord) and
+    the chord routine."""
+    from framelab_amd import geometry as G
+    if kind == "dz":
+        p = G.cone_para(24, 20, 6, N=(12, 12, 10), L=(2.0, 2.0, 2.0 * 10 * 0.8 / 12), SO=4.0, OD=4.0,
+                        Ly=G.full_fan_width(), Lz=G.full_cone_height())
+        assert abs(float(p.dz) - 0.8) < 1e-6
+    else:
+        p = G.cone_para(24, 20, 8, N=(12, 12, 12), SO=4.0, OD=4.0, Ly=G.full_fan_width(),
+                        Lz=G.full_cone_height(), kind="helix", zmax=0.5, rps=1.0, vtab=1.0, fps=8.0)
+        assert np.ptp(np.asarray(p.sd_z)) > 0
+    x = oracle.atx_cone_f32(np.ones(p.nv * p.nb * p.na, np.float32), p).astype(np.float64)
+    want = _cone_column_sums_fp64(p)
+    err = np.max(np.abs(x - want)) / np.max(want)
+    assert err <= 2e-5, err
