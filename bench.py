@@ -9,12 +9,12 @@ voxel-driven adjoint) of the whole volume, inputs resident in HBM.
 
 Prints ONE JSON line.  value = proj+backproj GUPS = 2*N_vox*n_views /
 (t_A + t_At) / 1e9 over the timed steps (FP32 plain path).  Beside it:
-  cgls      -- CGLS iterations/s of the FP32 plain path (Siddon + exact
-               adjoint, FP32 storage) and of the mixed path (FP16 storage of
-               the projector input and sinogram, FP32 accumulation, matched
-               adjoint), both device-resident with FP64 reductions and
-               graph-replayed iterations, and the mixed reconstruction's
-               relative L2 difference to the FP32 one;
+  cgls      -- 50 CGLS iterations of the FP32 plain path (Siddon + exact
+               adjoint, FP32 storage), of the precision change alone (FP16 /
+               BF16 storage), of the operator change alone (matched adjoint)
+               and of the mixed path (both), device-resident with FP64
+               reductions and graph-replayed iterations: iterations/s and the
+               relative L2 difference to the FP64 GPU solve;
   roofline  -- the dominant kernel's algorithmic bytes per launch over its
                measured duration, against 8 TB/s;
   framelet  -- 27-band 3-D framelet analysis and synthesis of the volume,
@@ -22,8 +22,13 @@ Prints ONE JSON line.  value = proj+backproj GUPS = 2*N_vox*n_views /
   batch     -- 64 uneven 2-D fan problems in persistent batched launches;
   c4        -- the 512^3, 720-view, 1024x768 configuration in BF16 storage:
                forward, exact and matched adjoint;
-  cpu_baseline -- the reference's own single-threaded CPU kernels on a
-               bounded sample of views.
+  cpu_baseline -- the oracle's restatement of the reference's CPU path on
+               a bounded sample (one view at full size, median of 5 views),
+               on 1 core and on every core, with the 1-core figure rescaled
+               to the reference's own kernels by a ratio measured in the
+               build container;
+  c1        -- the 2-D fan configuration (256^2, 360 x 512) timed in full on
+               the CPU restatement (A, A^T, 20 CGLS) and on the GPU.
 """
 from __future__ import annotations
 
@@ -58,42 +63,137 @@ def cpu_info():
     return model, cores
 
 
-def cpu_baseline(p, X, views=3):
-    """The reference's own single-threaded CPU kernels (oracle/_ref), or the
-    oracle's restatement when _ref is absent, on `views` views of the same
-    volume: Siddon forward + exact adjoint, reported as proj+backproj GUPS."""
+def _median_s(fn, runs=5):
+    import statistics
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return statistics.median(ts)
+
+
+def _calibration():
+    """Reference / restatement speed ratios on one core, measured in the build
+    container (tests/golden/cpu_calibration.json), or None."""
+    try:
+        with open(os.path.join(ROOT, "tests", "golden", "cpu_calibration.json")) as fh:
+            return json.load(fh)
+    except OSError:
+        return None
+
+
+def cpu_baseline(p, X, views=5):
+    """The oracle's C restatement of the reference's CPU path (the reference
+    code does not travel to this host) on a bounded sample of the workload:
+    one Siddon forward plus one exact adjoint of the full 256^3 volume for
+    one view, median over `views` distinct views, on 1 core (the reference
+    is single-threaded) and on every core of this process (OpenMP over
+    rays / voxels).  The restatement follows the reference's float
+    operations, and the reference / restatement ratio measured on one core
+    in the build container rescales the 1-core figure to the reference's."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
     import oracle as O
     from framelab_amd import geometry as G
-    sub = G.cone_para(p.na, p.nb, views, N=(p.nx, p.ny, p.nz), SO=4.0, OD=4.0,
-                      Ly=G.full_fan_width(), Lz=G.full_cone_height(),
-                      sd_phi=p.sd_phi[:: max(1, p.nv // views)][:views])
-    kind = "reference" if O.has_ref() else "port"
-    fwd = O.ref_siddon_cone if kind == "reference" else O.siddon_cone_f32
-    adj = O.ref_atx_cone_cpu if kind == "reference" else O.atx_cone_f32
-    O.set_threads(1)
-    t0 = time.perf_counter()
-    y = fwd(X, sub)
-    t1 = time.perf_counter()
-    adj(y, sub)
-    t2 = time.perf_counter()
     nvox = p.nx * p.ny * p.nz
-    gups = 2.0 * nvox * views / ((t1 - t0) + (t2 - t1)) / 1e9
+    picks = [int(k * p.nv / views) + 7 for k in range(views)]
+
+    def one_view(v):
+        return G.cone_para(p.na, p.nb, 1, N=(p.nx, p.ny, p.nz), SO=4.0, OD=4.0, Ly=G.full_fan_width(),
+                           Lz=G.full_cone_height(), sd_phi=np.array([p.sd_phi[v]], np.float32))
+
+    subs = [one_view(v) for v in picks]
     model, cores = cpu_info()
-    out = {"value": gups, "unit": "GUPS", "cores": 1, "kind": kind,
-           "sample": f"{views} of {p.nv} views at full 256^3 / 512x384 size, one forward + one adjoint "
-                     f"(t_A={t1 - t0:.2f}s, t_At={t2 - t1:.2f}s); host: {model}, {cores} logical CPUs"}
-    # the restatement (same algorithm, OpenMP over rays / voxels) on every core of this process
     ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    O.set_threads(ncpu)
-    t0 = time.perf_counter()
-    y = O.siddon_cone_f32(X, sub)
-    t1 = time.perf_counter()
-    O.atx_cone_f32(y, sub)
-    t2 = time.perf_counter()
+    out, times = {}, {}
+    for threads in (1, ncpu):
+        O.set_threads(threads)
+        tA, tAt = [], []
+        for sub in subs:
+            t0 = time.perf_counter()
+            y = O.siddon_cone_f32(X, sub)
+            t1 = time.perf_counter()
+            O.atx_cone_f32(y, sub)
+            t2 = time.perf_counter()
+            tA.append(t1 - t0)
+            tAt.append(t2 - t1)
+        a, at = float(np.median(tA)), float(np.median(tAt))
+        times[threads] = (a, at)
+        out[threads] = {"value": 2.0 * nvox / (a + at) / 1e9, "unit": "GUPS", "cores": threads, "kind": "port",
+                        "sample": f"one forward + one exact adjoint of the full 256^3 volume for one view of 512x384 "
+                                  f"cells, median over {views} views (t_A={a:.2f}s, t_At={at:.2f}s); host: {model}, "
+                                  f"{cores} logical CPUs"}
     O.set_threads(1)
-    out["all_cores"] = {"value": 2.0 * nvox * views / (t2 - t0) / 1e9, "unit": "GUPS", "cores": ncpu,
-                        "kind": "port", "sample": f"same sample, t_A={t1 - t0:.2f}s, t_At={t2 - t1:.2f}s"}
+    res = dict(out[1])
+    res["all_cores"] = out[ncpu]
+    cal = _calibration()
+    if cal:
+        k = cal["kernels"]
+        rf = k["cone_siddon_forward_256cube_1view"]["reference_over_restatement"]
+        ra = k["cone_exact_adjoint_256cube_1view"]["reference_over_restatement"]
+        tA1, tAt1 = times[1]
+        res["reference_equivalent_1core"] = {
+            "value": 2.0 * nvox / (tA1 * rf + tAt1 * ra) / 1e9, "unit": "GUPS",
+            "ratios": {"forward": rf, "exact_adjoint": ra}, "calibrated_on": cal["host"]}
+    return res
+
+
+def cpu_c1(dev):
+    """The 2-D fan configuration timed in full on the CPU restatement: 256^2
+    image of 10 seeded ellipses, 360 views x 512 bins, SO 4, SD 8; Siddon
+    forward A, exact pixel-driven adjoint A^T (medians of 5), and 20 CGLS
+    iterations on the noiseless sinogram, on 1 core and on every core; the
+    same workload on the GPU beside it."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import torch
+    import oracle as O
+    from framelab_amd import geometry as G
+    from framelab_amd.phantoms import ellipses_2d
+    from framelab_amd.projector import Projector
+    from framelab_amd.solvers import cgls
+    p = G.fan_para(512, 360, N=(256, 256), SO=4.0, OD=4.0, Ly=G.full_fan_width())
+    X = ellipses_2d(256, 10, seed=1).astype(np.float32)
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    out = {"workload": "fan 256^2, 360 views x 512 bins, SO=4 SD=8: A, A^T (medians of 5) and 20 CGLS iterations"}
+    y = O.siddon_fan_f32(X, p)
+    for threads in (1, ncpu):
+        O.set_threads(threads)
+        tA = _median_s(lambda: O.siddon_fan_f32(X, p))
+        tAt = _median_s(lambda: O.atx_fan_f32(y, p))
+        A = lambda v: O.siddon_fan_f32(v.astype(np.float32), p).astype(np.float64)
+        At = lambda w: O.atx_fan_f32(w.astype(np.float32), p).astype(np.float64)
+        b = At(y)
+        t0 = time.perf_counter()
+        O.cgls(A, At, b, np.zeros_like(b), 0.0, 20)
+        tc = time.perf_counter() - t0
+        out[f"cpu_{threads}_cores"] = {"t_A_s": tA, "t_At_s": tAt, "cgls_20_s": tc, "cgls_iters_per_s": 20 / tc}
+    O.set_threads(1)
+    P = Projector(p)
+    x = torch.from_numpy(X).to(dev)
+    yg = P.forward(x)
+    P.adjoint(yg)
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    tg = []
+    for _ in range(5):
+        evs[0].record()
+        P.forward(x, out=yg)
+        evs[1].record()
+        xb = P.adjoint(yg)
+        evs[2].record()
+        torch.cuda.synchronize()
+        tg.append((evs[0].elapsed_time(evs[1]), evs[1].elapsed_time(evs[2])))
+    b = P.adjoint(yg)
+    cgls(P, b, torch.zeros_like(b), 0.0, 20, graph=True, log=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cgls(P, b, torch.zeros_like(b), 0.0, 20, graph=True, log=False)
+    torch.cuda.synchronize()
+    tc = time.perf_counter() - t0
+    out["gpu"] = {"t_A_ms": float(np.median([t[0] for t in tg])), "t_At_ms": float(np.median([t[1] for t in tg])),
+                  "cgls_20_s": tc, "cgls_iters_per_s": 20 / tc}
+    del xb
     return out
 
 
@@ -130,6 +230,77 @@ def time_cgls(P, b, u0, iters, **kw):
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
     return (t - t_init) / iters, u
+
+
+def cgls_leg(P, x, iters):
+    """CGLS iterations/s at 256^3 (graph-replayed iterations, FP64 dots), each
+    path against the FP64 GPU solve (FP64 vectors over the FP64 projector
+    pair) after the same number of iterations on the same noiseless data:
+      fp32_plain    -- FP32 storage, Siddon + exact adjoint (the reference's pair);
+      fp16_exact / bf16_exact -- the same operators with FP16 / BF16 storage of
+                       p and A p (the precision change alone);
+      fp32_matched  -- FP32 storage with the matched adjoint (the operator
+                       change alone);
+      mixed         -- FP16 storage and the matched adjoint.
+    precision_only_speedup and operator_swap_speedup separate the two
+    sources of mixed_vs_fp32_speedup.  The mixed iteration's kernel budget
+    is the plain iteration's time over 3 (the 3x target)."""
+    import torch
+    from framelab_amd.projector import ADJ_MATCHED
+    from framelab_amd.solvers import cgls_f64
+    x64 = x.double()
+    b64 = P.adjoint_f64(P.forward_f64(x64))
+    cgls_f64(P, b64, torch.zeros_like(x64), iters=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    u64, g64, _ = cgls_f64(P, b64, torch.zeros_like(x64), iters=iters)
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t0
+    ref = u64.float()
+    del b64, x64
+    b_exact = P.adjoint(P.forward(x))
+    b_matched = P.adjoint(P.forward(x), kind=ADJ_MATCHED)
+    u0 = torch.zeros_like(x)
+    legs = {
+        "fp32_plain": (b_exact, {}),
+        "fp16_exact": (b_exact, {"storage": torch.float16}),
+        "bf16_exact": (b_exact, {"storage": torch.bfloat16}),
+        "fp32_matched": (b_matched, {"matched": True}),
+        "mixed": (b_matched, {"matched": True, "storage": torch.float16}),
+    }
+    out = {"iters": iters, "fp64_iters_per_s": iters / t64,
+           "fp64_vs_phantom_rel_l2": float((ref - x).norm() / x.norm())}
+    for name, (b, kw) in legs.items():
+        s, u = time_cgls(P, b, u0, iters, graph=True, **kw)
+        out[name] = {"iters_per_s": 1.0 / s, "rel_l2_vs_fp64": float((u - ref).norm() / ref.norm())}
+        log(f"bench: cgls {name}: {1.0 / s:.2f} it/s, rel L2 vs FP64 {out[name]['rel_l2_vs_fp64']:.2e}")
+    plain = out["fp32_plain"]["iters_per_s"]
+    out["precision_only_speedup"] = out["fp16_exact"]["iters_per_s"] / plain
+    out["operator_swap_speedup"] = out["fp32_matched"]["iters_per_s"] / plain
+    out["mixed_vs_fp32_speedup"] = out["mixed"]["iters_per_s"] / plain
+    out["mixed_rel_l2_vs_fp64"] = out["mixed"]["rel_l2_vs_fp64"]
+    # one mixed iteration: its forward and adjoint kernels timed alone, the
+    # rest (vector updates, reductions) by difference
+    xl = x.to(torch.float16)
+    yl = torch.empty(P.sino_shape, dtype=torch.float16, device=x.device)
+    xo = torch.empty_like(x)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    P.forward(xl, out=yl)
+    P.adjoint(yl, out=xo, kind=ADJ_MATCHED)
+    ev[0].record()
+    P.forward(xl, out=yl)
+    ev[1].record()
+    P.adjoint(yl, out=xo, kind=ADJ_MATCHED)
+    ev[2].record()
+    torch.cuda.synchronize()
+    t_it = 1000.0 / out["mixed"]["iters_per_s"]
+    tf, ta = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
+    out["mixed_iteration_ms"] = {"total": t_it, "forward_fp16": tf, "matched_adjoint_fp16": ta,
+                                 "vector_updates_and_reductions": t_it - tf - ta,
+                                 "budget_for_3x": 1000.0 / (3.0 * plain)}
+    out["paths"] = {"fp32_plain": "FP32 storage, Siddon + exact adjoint", "mixed": "FP16 storage of p and A p, FP32 "
+                    "accumulation, Siddon + matched adjoint", "reductions": "FP64 on the GPU, graph-replayed iterations"}
+    return out
 
 
 def batch_problems(count=64, seed=5):
@@ -330,10 +501,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cgls-iters", type=int, default=8)
+    ap.add_argument("--cgls-iters", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-cgls", action="store_true", help="skip the CGLS leg")
-    ap.add_argument("--cpu-views", type=int, default=3)
+    ap.add_argument("--cpu-views", type=int, default=5)
+    ap.add_argument("--no-c1", action="store_true", help="skip the 2-D fan CPU/GPU leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 2D leg")
     ap.add_argument("--no-c4", action="store_true", help="skip the 512^3 BF16 leg")
     ap.add_argument("--profile", action="store_true",
@@ -413,7 +585,7 @@ def main():
     wall = time.perf_counter() - t0
     tA = [e[0].elapsed_time(e[1]) for e in evs]
     tAt = [e[1].elapsed_time(e[2]) for e in evs]
-    mA, mAt = float(np.mean(tA)), float(np.mean(tAt))
+    mA, mAt = float(np.median(tA)), float(np.median(tAt))
     gups = 2.0 * nvox * nv / ((mA + mAt) * 1e-3) / 1e9
     log(f"bench: A {mA:.2f} ms, At(exact) {mAt:.2f} ms, wall/step {wall / args.steps * 1e3:.2f} ms")
 
@@ -470,28 +642,7 @@ def main():
 
     if not args.no_cgls:
         log("bench: cgls")
-        b_rhs = P.adjoint(P.forward(x))
-        u0 = torch.zeros_like(x)
-        it = args.cgls_iters
-        s32, u32 = time_cgls(P, b_rhs, u0, it, graph=True)
-        bm = P.adjoint(P.forward(x), kind=ADJ_MATCHED)
-        smx, umx = time_cgls(P, bm, u0, it, graph=True, matched=True, storage=torch.float16)
-        s16, u16 = time_cgls(P, b_rhs, u0, it, graph=True, storage=torch.float16)
-        rel = float((umx - u32).norm() / u32.norm())
-        rel16 = float((u16 - u32).norm() / u32.norm())
-        result["cgls"] = {
-            "iters": it,
-            "fp32_plain_iters_per_s": 1.0 / s32,
-            "mixed_iters_per_s": 1.0 / smx,
-            "mixed_vs_fp32_speedup": s32 / smx,
-            "mixed_rel_l2_vs_fp32": rel,
-            "fp16_exact_adjoint_iters_per_s": 1.0 / s16,
-            "fp16_exact_adjoint_rel_l2_vs_fp32": rel16,
-            "fp32_plain": "FP32 storage, Siddon + exact adjoint, FP64 dots, graph-replayed",
-            "mixed": "FP16 storage of p and A p, FP32 accumulation, Siddon + matched adjoint, FP64 dots, "
-                     "graph-replayed",
-        }
-        log(f"bench: cgls fp32 {1 / s32:.2f} it/s, mixed {1 / smx:.2f} it/s, rel {rel:.2e}")
+        result["cgls"] = cgls_leg(P, x, args.cgls_iters)
 
     if not args.no_batch:
         log("bench: batch")
@@ -508,6 +659,9 @@ def main():
     if not args.no_cpu:
         log("bench: cpu baseline")
         result["cpu_baseline"] = cpu_baseline(p, Xh, views=args.cpu_views)
+        if not args.no_c1:
+            log("bench: 2-D fan on CPU and GPU")
+            result["c1"] = cpu_c1(dev)
     print(json.dumps(result), flush=True)
 
 

@@ -7,6 +7,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "../../include/framelab_hip.h"
@@ -202,11 +203,18 @@ extern "C" int fl_plan_create(fl_plan* out, const fl_geometry* g)
                 if (v < 0 || v >= g->nv) return fl_fail(FL_ERR_INVALID, "id_Y entry %d outside [0,nv)", v);
             }
     } else {
-        if (g->nt != 1) return fl_fail(FL_ERR_INVALID, "id_Y/Nv required when nt > 1");
-        tmp = g->nv;
-        Nv[0] = g->nv;
-        idY.resize(g->nv);
-        for (int v = 0; v < g->nv; v++) idY[v] = v;
+        // frame lists from view -> frame (views ascending within each
+        // frame), the lists setPara builds (ConeBeamScanner/setPara.m:59-80)
+        std::fill(Nv.begin(), Nv.end(), 0);
+        for (int v = 0; v < g->nv; v++) Nv[view_frame[v]]++;
+        tmp = 0;
+        for (int it = 0; it < g->nt; it++) tmp = Nv[it] > tmp ? Nv[it] : tmp;
+        idY.assign((size_t)tmp * g->nt, 0);
+        std::vector<int> fill(g->nt, 0);
+        for (int v = 0; v < g->nv; v++) {
+            const int it = view_frame[v];
+            idY[(size_t)it * tmp + fill[it]++] = v;
+        }
     }
     if (idY.empty()) idY.push_back(0);
     // The adjoint reads the frame-list entry (it, j) from the slot the
@@ -384,13 +392,20 @@ extern "C" int fl_plan_create(fl_plan* out, const fl_geometry* g)
     return fl_ok();
 }
 
-extern "C" int fl_plan_destroy(fl_plan p)
+// Frees a plan without touching this thread's error status (the drop-in
+// entry points release their plans on every path, failed ones included).
+static void plan_release(fl_plan_s* p)
 {
-    if (!p) return fl_ok();
+    if (!p) return;
     fl_plan_release_workspace(p);
     fl_plan_release_f64(p);
     plan_free_arrays(p);
     delete p;
+}
+
+extern "C" int fl_plan_destroy(fl_plan p)
+{
+    plan_release(p);
     return fl_ok();
 }
 
@@ -654,7 +669,7 @@ struct DevBuf {
 
 struct PlanGuard {
     fl_plan p = nullptr;
-    ~PlanGuard() { fl_plan_destroy(p); }
+    ~PlanGuard() { plan_release(p); }
 };
 
 // Frame -> view lists to view -> frame and output slot order.
@@ -798,6 +813,7 @@ static void cone_fwd_dropin(float* X, float* y, float* yNorm, float* sd_phi, flo
         }
         g.id_X = idX.data();
         g.slot_view = slots.data();
+        g.id_Y = id_Y; g.Nv = Nv; g.tmp_size = tmp_size;
     }
     run_forward_host(g, X, y, yNorm, false);
 }

@@ -44,18 +44,14 @@ def _geom(kind):
 
 def _frames_sum_oracle(fn, arr, p, per_frame_in, cone):
     """The oracle's FP64 pair works on one frame: apply it per frame (views
-    of frame f only) and assemble."""
-    if p.nt == 1:
-        return fn(arr, p, cone=cone)
-    import dataclasses
+    of frame f only, a one-frame geometry of those angles) and assemble."""
+    from framelab_amd import geometry as G
     out = []
     for f in range(p.nt):
         views = np.nonzero(np.asarray(p.id_X) == f)[0]
-        sub = dataclasses.replace(p, nt=1, nv=int(views.size), sd_phi=p.sd_phi[views],
-                                  sd_z=p.sd_z[views], cos_phi=p.cos_phi[views], sin_phi=p.sin_phi[views],
-                                  id_X=np.zeros(views.size, np.int32),
-                                  id_Y=np.arange(views.size, dtype=np.int32),
-                                  Nv=np.array([views.size], np.int32), tmp_size=int(views.size))
+        sub = G.cone_para(p.na, p.nb, int(views.size), N=(p.nx, p.ny, p.nz), SO=4.0, OD=4.0,
+                          Ly=G.full_fan_width(), Lz=G.full_cone_height(), sd_phi=p.sd_phi[views])
+        assert np.array_equal(sub.y_det, p.y_det) and np.array_equal(sub.z_det, p.z_det)
         out.append((views, fn(per_frame_in(arr, f, views), sub, cone=cone)))
     return out
 
@@ -301,8 +297,8 @@ def test_forward_norm_checks_and_dropin_ynorm(gpu, oracle):
                          p.ny, p.nz, 1, p.na, p.nb, p.nv, p.tmp_size, 4)
     _lib.check_last()
     yt = P.forward(torch.from_numpy(X.reshape(P.vol_shape)).to(gpu))
-    assert bits_equal(y, yt.cpu().numpy())
-    assert bits_equal(yn, P.forward_norm(yt).cpu().numpy())
+    assert bits_equal(y, yt.cpu().numpy().ravel())
+    assert bits_equal(yn, P.forward_norm(yt).cpu().numpy().ravel())
     c, s = oracle.trig_f32(p.sd_phi)
     c, s = c.astype(np.float64), s.astype(np.float64)
     ya, zb = np.asarray(p.y_det, np.float64), np.asarray(p.z_det, np.float64)

@@ -345,3 +345,31 @@ def test_cone_adjoint_column_sums(oracle, kind):
     want = _cone_column_sums_fp64(p)
     err = np.max(np.abs(x - want)) / np.max(want)
     assert err <= 2e-5, err
+
+
+def test_find_l_3d_edge_miss(oracle):
+    """A defect of the reference's chord routine (find_l_3d.cpp:94-182),
+    reproduced bit for bit by the exact adjoint: a ray that leaves a box
+    within rounding of an edge gets chord 0.  Its exit candidates on the two
+    faces meeting at that edge both fail their `<=` tests by an ulp
+    (coordinates -0.50000066 and -0.5000004 against the half side 0.5), so
+    only the entry point is found.  Ray of view 9, column 88, row 16 of a
+    64^3 cone (128 x 96 cells, 30 views, SO 4, SD 8) through voxel
+    (x, y, z) = (17, 20, 1): FP64 chord 1.118, reference and oracle 0.  This
+    is why the Siddon / exact-chord pair's Gaussian dot test sits near 1e-3
+    while the Siddon / matched pair's is near 1e-7."""
+    from framelab_amd import geometry as G
+    f = np.float32
+    p = G.cone_para(128, 96, 30, N=(64, 64, 64), SO=4.0, OD=4.0, Ly=G.full_fan_width(), Lz=G.full_cone_height())
+    v, ia, ib = 9, 88, 16
+    c, s = f(p.cos_phi[v]), f(p.sin_phi[v])
+    x1, y1, z1 = c * (-f(p.SO)), s * (-f(p.SO)), f(p.sd_z[v])
+    yd = f(p.y_det[ia])
+    x2, y2, z2 = c * f(p.OD) - s * yd, s * f(p.OD) + c * yd, f(p.z_det[ib]) + z1
+    xc, yc, zc = f(17 + 0.5 - 32), f(20 + 0.5 - 32), f(1 + 0.5 - 32)
+    args = tuple(float(t) for t in (x1, y1, z1, x2, y2, z2)) + (1.0, 1.0, 1.0) + (float(xc), float(yc), float(zc))
+    l32 = oracle.find_l_3d(*args)
+    l64 = oracle.chord_box_f64(np.array(args[0:3]), np.array(args[3:6]), np.array(args[9:12]), np.full(3, 0.5))
+    assert l32 == 0.0 and abs(l64 - 1.118028) < 1e-5
+    if oracle.has_ref():
+        assert oracle.ref().ref_find_l_3d(*args) == 0.0
