@@ -117,6 +117,8 @@ SIGNATURES = {
     "fl_grad_data": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(C.c_double), C.c_int, _vp]),
     "fl_grad_framelet": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_float,
                                    C.c_float, C.c_int, _vp, _vp, C.POINTER(C.c_double), _vp]),
+    "fl_grad_framelet_noalloc": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_float,
+                                           C.c_float, C.c_int, _vp, _vp, _vp, _vp]),
     "fl_plan_autotune": (C.c_int, [_vp, _ip, _fp]),
     "fl_plan_set_tile": (C.c_int, [_vp, C.c_int]),
     "fl_forward_norm": (C.c_int, [_vp, _vp, _vp, _vp]),
@@ -141,6 +143,7 @@ SIGNATURES = {
 FL_FRAME_HAAR, FL_FRAME_LINEAR, FL_FRAME_CUBIC = 0, 1, 3
 FL_SOLVE_GRAPH, FL_SOLVE_MATCHED, FL_SOLVE_LOWP_BF16, FL_SOLVE_LOWP_F16 = 1, 2, 4, 8
 FL_GRAD_EXACT_ADJOINT = 1
+FL_NOALLOC_PARTIALS = 1 << 21   # doubles in fl_grad_framelet_noalloc's static scratch
 
 
 def lib():

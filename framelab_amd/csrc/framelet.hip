@@ -302,8 +302,14 @@ __global__ __launch_bounds__(256) void k_iwt3z(const CT* __restrict__ C, XT* u, 
     __shared__ int poff[P];            // halo position -> wrapped offset in a slice
     const size_t nxy = (size_t)nx * ny, nvox = nxy * nz;
     const int tilesx = (nx + IW_TX - 1) / IW_TX;
-    const int x0 = (int)(blockIdx.x % tilesx) * IW_TX, y0 = (int)(blockIdx.x / tilesx) * IW_TY;
-    const int z0 = (int)blockIdx.y * ZC;
+    // consecutive tiles (x fastest, then y, then z) on one XCD: the
+    // workgroups an XCD runs together are neighbours, so the cache lines
+    // their halos share (partial lines at the tile edges, the y and z halo
+    // rows) are fetched into that XCD's L2 once
+    const uint32_t lin = fl_xcd_remap(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const uint32_t tile = lin % gridDim.x;
+    const int x0 = (int)(tile % tilesx) * IW_TX, y0 = (int)(tile / tilesx) * IW_TY;
+    const int z0 = (int)(lin / gridDim.x) * ZC;
     const int tid = threadIdx.x;
     for (int p = tid; p < P; p += 256) poff[p] = wrapi(y0 - H + p / PW, ny) * nx + wrapi(x0 - H + p % PW, nx);
     size_t zoff[WZ];   // input slices z0 - H .. z0 + ZC - 1 + H (wrapped)

@@ -911,3 +911,65 @@ extern "C" int fl_framelet_shrink(int nbands, int level, long long band_elems, v
     HIPCHK(hipGetLastError());
     return fl_ok();
 }
+
+// ------------------------------------- framelet gradient, no allocation --
+//
+// fl_grad_framelet_noalloc: the FP64 loss partials live in `scratch`, a
+// file-scope device array of FL_NOALLOC_PARTIALS doubles (16 MiB) that each
+// GPU holds until the process ends, instead of the grow-only buffer
+// fl_grad_framelet allocates; the loss goes to a device pointer, so the call
+// neither allocates nor synchronises.  Every partial is written by its
+// workgroup before the fixed-order sum reads it, so nothing needs clearing.
+// Calls that overlap on one GPU share the array and race.
+
+static __device__ __align__(256) double scratch[FL_NOALLOC_PARTIALS];
+
+namespace {
+std::mutex g_noalloc_mu;
+std::vector<double*> g_noalloc_ptr;   // per device: the device address of `scratch`
+
+__global__ void k_copy_double(const double* __restrict__ a, double* __restrict__ b) { *b = *a; }
+
+// device address of `scratch` on the current device (resolved once per device)
+int noalloc_scratch(double** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fl_hip_fail(e, "hipGetDevice");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if ((int)g_noalloc_ptr.size() <= dev) g_noalloc_ptr.resize(dev + 1, nullptr);
+    if (!g_noalloc_ptr[dev]) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) return fl_hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+        g_noalloc_ptr[dev] = (double*)ptr;
+    }
+    *out = g_noalloc_ptr[dev];
+    return FL_OK;
+}
+}  // namespace
+
+extern "C" int fl_grad_framelet_noalloc(int dim, int kind, int level, int nx, int ny, int nz, const float* d_x,
+                                        float lam, float eps, int with_lowpass, float* d_g, float* d_work,
+                                        double* d_loss, void* stream)
+{
+    if (!frame_ok(dim, kind, nx, ny, nz) || !d_x || !d_g || !d_work || !(eps > 0) || level < 1 || level > 8)
+        return fl_fail(FL_ERR_INVALID, "bad framelet gradient arguments");
+    FlFrameDesc f = {dim, kind, nx, ny, dim == 3 ? nz : 1};
+    const unsigned nbk = fl_frame_blocks(f);
+    const size_t need = (size_t)nbk * level + 1;
+    if (need > (size_t)FL_NOALLOC_PARTIALS)
+        return fl_fail(FL_ERR_INVALID, "volume too large for the no-allocation gradient: %zu partials of the %d in its "
+                                       "static scratch (about %d voxels x levels)", need, FL_NOALLOC_PARTIALS,
+                       (FL_NOALLOC_PARTIALS - 1) * 256);
+    double* part = nullptr;
+    int rc = noalloc_scratch(&part);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = grad_framelet_run(f, level, d_x, lam, eps, with_lowpass, d_g, d_work, part, nbk, st))) return rc;
+    if (d_loss) {
+        hipLaunchKernelGGL(k_copy_double, dim3(1), dim3(1), 0, st, part + (size_t)nbk * level, d_loss);
+        HIPCHK(hipGetLastError());
+    }
+    return fl_ok();
+}

@@ -253,6 +253,21 @@ int fl_grad_framelet(int dim, int kind, int level, int nx, int ny, int nz, const
                      float lam, float eps, int with_lowpass, float *d_g, float *d_work,
                      double *loss, void *stream);
 
+/* fl_grad_framelet for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments and results as
+ * fl_grad_framelet except the loss, which goes to the device pointer d_loss
+ * (one double, optional), so the call neither allocates nor synchronises.
+ * Its FP64 partials live in a static device array of FL_NOALLOC_PARTIALS
+ * doubles (16 MiB, held by each GPU the library runs on until the process
+ * ends): ceil(nx*ny*nz/256)*level + 1 must not exceed it (FL_ERR_INVALID
+ * otherwise), i.e. about 5.4e8 voxels x levels.  Calls that overlap on one
+ * GPU share that array and race: order them on one stream. */
+#define FL_NOALLOC_PARTIALS (1 << 21)
+int fl_grad_framelet_noalloc(int dim, int kind, int level, int nx, int ny, int nz, const float *d_x,
+                             float lam, float eps, int with_lowpass, float *d_g, float *d_work,
+                             double *d_loss, void *stream);
+
 /* ------------------------------------------------------------- batches
  * Many independent 2D fan-beam problems of uneven sizes (one frame each),
  * each operator one persistent launch over all of them.  Volumes and
