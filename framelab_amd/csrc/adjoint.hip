@@ -16,6 +16,12 @@
 #include "fl_geom.h"
 #include "fl_fan.h"
 
+// Workgroups per CU the register budget is sized for (a build-time tuning
+// knob; the default is the measured optimum at 256^3).
+#ifndef FL_EXACT_WG_PER_CU
+#define FL_EXACT_WG_PER_CU 5
+#endif
+
 template <typename YT, typename XT>
 __global__ __launch_bounds__(256) void k_adj_fan(const YT* __restrict__ Y, XT* __restrict__ X, AdjGeom g)
 {
@@ -59,7 +65,7 @@ __device__ __forceinline__ int wave_max(int v) { return __ockl_wfred_max_i32(v);
 // Five workgroups per CU (the LDS tables take 26.6 KB each; the register
 // budget that allows five is the measured optimum at 256^3).
 template <typename YT, typename XT, int KZ>
-__global__ __launch_bounds__(256, 5) void k_adj_cone_zc(const YT* __restrict__ Y, XT* __restrict__ X, AdjGeom g)
+__global__ __launch_bounds__(256, FL_EXACT_WG_PER_CU) void k_adj_cone_zc(const YT* __restrict__ Y, XT* __restrict__ X, AdjGeom g)
 {
     __shared__ float c_a[FP_A], c_b[FP_A], c_sba[FP_A], c_sab[FP_A];
     __shared__ float c_ra[FP_A], c_rb[FP_A];

@@ -24,6 +24,12 @@
 #include "fl_chord.h"
 #include "fl_fan.h"
 
+// Workgroups per CU the register budget is sized for (a build-time tuning
+// knob; the default is the measured optimum at 256^3).
+#ifndef FL_MATCHED_WG_PER_CU
+#define FL_MATCHED_WG_PER_CU 5
+#endif
+
 
 
 // Per (ray, voxel) pair of one view: the Siddon segment of the ray inside
@@ -65,7 +71,7 @@ __device__ __forceinline__ int mwave_max(int v) { return __ockl_wfred_max_i32(v)
 // in order, columns outer; FP32 accumulation.
 // Five workgroups per CU (measured optimum of the register budget).
 template <typename YT, typename XT, int KZ, bool DZ1>
-__global__ __launch_bounds__(256, 5) void k_adj_matched_cone(const YT* __restrict__ Y, XT* __restrict__ X,
+__global__ __launch_bounds__(256, FL_MATCHED_WG_PER_CU) void k_adj_matched_cone(const YT* __restrict__ Y, XT* __restrict__ X,
                                                          FwdGeom f, AdjGeom g)
 {
     constexpr int BZ = 4 * KZ;

@@ -126,3 +126,33 @@ def test_benchmark_detector_extents():
     """Full fan covers the circumscribed circle r=sqrt(2): half-angle asin(sqrt2/4)."""
     assert abs(G.full_fan_width() - 6.047432) < 1e-5
     assert abs(G.full_cone_height() - 6.187673) < 1e-5
+
+
+def test_matlab_class_fixes_shipped():
+    """The MATLAB class methods that replace the reference's defective ones
+    are shipped (framelab_amd/matlab/classes, copied over the reference's
+    matlab/classes): their signatures return what the callers use --
+    softThreshold returns the shrunk expansion and solveSplitBregman keeps
+    it; FanBeamScanner.applyAdjoint takes the parameters from data.scanner;
+    plus/minus/mtimes build results from the class's own properties.  (No
+    MATLAB here: checked as text.)"""
+    import os
+    root = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "framelab_amd", "matlab",
+                        "classes")
+    want = {
+        "+Operators/@FanBeamScanner/applyAdjoint.m": ("function Aty = applyAdjoint(this, data)", "data.scanner.para"),
+        "+DataTypes/@FrameletExpansion/plus.m": ("function c = plus(a, b)", "a.frameletSystem"),
+        "+DataTypes/@FrameletExpansion/minus.m": ("function c = minus(a, b)", "a.frameletSystem"),
+        "+DataTypes/@FrameletExpansion/mtimes.m": ("function c = mtimes(a, b)", "frameletSystem"),
+        "+DataTypes/@FrameletExpansion/softThreshold.m": ("function obj = softThreshold(obj, tau)", "max(abs(a) - tau, 0)"),
+        "+DataTypes/@FrameletExpansion/private/combineBands.m": ("function C = combineBands(A, B, op)", "cellfun"),
+        "+Optimizers/@SplitBregman/solveSplitBregman.m": ("function u = solveSplitBregman(obj)",
+                                                          "alpha = alpha.softThreshold(obj.lam / obj.mu);"),
+    }
+    for rel, (sig, body) in want.items():
+        with open(os.path.join(root, rel)) as fh:
+            text = fh.read()
+        assert text.startswith(sig), rel
+        assert body in text, rel
+        code = "\n".join(l for l in text.splitlines() if not l.lstrip().startswith("%"))
+        assert "nLevel" not in code and "frameletType" not in code
