@@ -304,6 +304,39 @@ def cgls_leg(P, x, iters):
     return out
 
 
+def precision_leg(P, x, outer=30, inner=3):
+    """Split Bregman with the 27-band 3-D linear frame (mu 1, lam 1e-3,
+    `outer` x `inner`, graph-replayed CGLS with the matched adjoint, FP64
+    reductions) with the projector input and sinogram stored in BF16, in
+    FP16 and in FP32, each iterate against the FP64 solve's (FP64 pair and
+    FP64 framelets) after every outer step, on the 256^3 / 360-view
+    geometry (an FP64 solve at 512^3 / 720 views takes minutes)."""
+    import torch
+    from framelab_amd.solvers import split_bregman, split_bregman_f64
+    f64 = P.forward_f64(x.double())
+    f32 = f64.float()
+    u0 = torch.zeros_like(x)
+    out = {"workload": "cone 256^3, 360 views, 512x384; split Bregman, 27-band linear frame, mu=1, lam=1e-3, "
+                       f"{outer} outer x {inner} inner, matched adjoint; rel L2 per outer iterate vs FP64"}
+    t0 = time.perf_counter()
+    _, _, ref = split_bregman_f64(P, "linear", f64, u0.double(), 1e-3, 1.0, outer, inner, history=True)
+    torch.cuda.synchronize()
+    out["fp64_seconds"] = time.perf_counter() - t0
+    rnorm = ref.flatten(1).norm(dim=1)
+    for name, dt in (("fp32", torch.float32), ("bf16", torch.bfloat16), ("fp16", torch.float16)):
+        t0 = time.perf_counter()
+        _, _, h = split_bregman(P, "linear", f32, u0, 1e-3, 1.0, outer, inner, graph=True, matched=True, storage=dt,
+                                history=True)
+        torch.cuda.synchronize()
+        out[f"{name}_seconds"] = time.perf_counter() - t0
+        errs = ((h.double() - ref).flatten(1).norm(dim=1) / rnorm).tolist()
+        out[f"{name}_rel_l2_vs_fp64"] = errs
+        log(f"bench: split Bregman {name} storage vs FP64: outer 1 {errs[0]:.2e}, outer {outer} {errs[-1]:.2e}, "
+            f"max {max(errs):.2e}")
+        del h
+    return out
+
+
 def batch_problems(count=64, seed=5):
     """64 independent fan problems of seeded uneven sizes: image side from
     {128,192,256,384,512}, views U[90,720], bins U[256,1024], 6-15 ellipses,
@@ -509,6 +542,7 @@ def main():
     ap.add_argument("--no-c1", action="store_true", help="skip the 2-D fan CPU/GPU leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 2D leg")
     ap.add_argument("--no-c4", action="store_true", help="skip the 512^3 BF16 leg")
+    ap.add_argument("--no-precision", action="store_true", help="skip the split-Bregman precision leg")
     ap.add_argument("--profile", action="store_true",
                     help="counter run: one forward, one exact and one matched adjoint at 256^3, one "
                          "framelet analysis and synthesis (27 bands, FP32), then a "
@@ -644,6 +678,10 @@ def main():
     if not args.no_cgls:
         log("bench: cgls")
         result["cgls"] = cgls_leg(P, x, args.cgls_iters)
+
+    if not args.no_precision:
+        log("bench: precision (split Bregman vs FP64)")
+        result["precision"] = precision_leg(P, x)
 
     if not args.no_batch:
         log("bench: batch")

@@ -98,6 +98,8 @@ SIGNATURES = {
     "fl_siddon_trace": (C.c_int, [_vp, C.c_int, _ip, C.c_int, _ip, _fp, _ip]),
     "fl_forward_f64": (C.c_int, [_vp, _vp, _vp, _vp]),
     "fl_adjoint_f64": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "fl_split_bregman_f64": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_double, C.c_double, C.c_int, C.c_int,
+                                       C.POINTER(C.c_double), _vp, _vp]),
     "fl_cgls_f64": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_int, C.c_double, C.POINTER(C.c_double),
                               C.POINTER(C.c_int), _vp]),
     "fl_framelet_num_bands": (C.c_int, [C.c_int, C.c_int]),
@@ -108,7 +110,7 @@ SIGNATURES = {
     "fl_cgls": (C.c_int, [_vp, _vp, _vp, C.c_float, C.c_int, C.c_double, C.c_int,
                           C.POINTER(C.c_double), C.POINTER(C.c_int), _vp]),
     "fl_split_bregman": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.c_float, C.c_float, C.c_int, C.c_int,
-                                   C.c_double, C.c_int, C.POINTER(C.c_double), _vp]),
+                                   C.c_double, C.c_int, C.POINTER(C.c_double), _vp, _vp]),
     "fl_vec_cgls_init": (C.c_int, [C.c_longlong, _vp, _vp, _vp, C.c_float, _vp, _vp, C.POINTER(C.c_double), _vp]),
     "fl_vec_cgls_pq": (C.c_int, [C.c_longlong, _vp, _vp, C.c_float, C.POINTER(C.c_double), _vp]),
     "fl_vec_cgls_update": (C.c_int, [C.c_longlong, _vp, _vp, _vp, _vp, C.c_double, C.POINTER(C.c_double), _vp]),

@@ -17,7 +17,9 @@
 //    coefficient is the forward's (to FP64 rounding).  Candidate cells come
 //    from the voxel's shadow window on the detector (a 0.05-cell margin).
 //  * CGLS (matlab/classes/+Optimizers/@CGLS/solveCGLS.m:7-30) with every
-//    vector in double and fixed-order reductions.
+//    vector in double and fixed-order reductions, and split Bregman
+//    (solveSplitBregman.m:16-41) over it with a one-level framelet
+//    analysis / synthesis in double.
 //
 // No reference counterpart: the reference has FP32 projectors only.
 #include <hip/hip_runtime.h>
@@ -31,6 +33,7 @@
 #include "fl_capi_internal.h"
 #include "fl_chord.h"
 #include "fl_common.h"
+#include "fl_framelet.h"
 
 namespace {
 
@@ -446,6 +449,242 @@ extern "C" int fl_adjoint_f64(fl_plan p, const double* d_y, double* d_x, void* s
     return fl_ok();
 }
 
+namespace {
+
+// FP64 CG workspace (allocated per call: a reference path)
+struct Cg64Ws {
+    double *r = nullptr, *p = nullptr, *q = nullptr, *sino = nullptr, *part = nullptr;
+    Cg64* state = nullptr;
+};
+
+int cg64_alloc(DevMem& m, Cg64Ws& w, size_t n, size_t ns)
+{
+    int rc;
+    if ((rc = m.get(&w.r, n, "r")) || (rc = m.get(&w.p, n, "p")) || (rc = m.get(&w.q, n, "q")) ||
+        (rc = m.get(&w.sino, ns, "sinogram")) || (rc = m.get(&w.part, NP64 + 8, "partials")) ||
+        (rc = m.get(&w.state, 1, "state")))
+        return rc;
+    return FL_OK;
+}
+
+// `iters` CG steps on (A^T A + lam I) u = b from the u given (solveCGLS.m:7-30),
+// FP64 pair; log (device, iters+1 doubles) optional
+int cg64_run(const fl_plan_s* p, Cg64Ws& w, const double* b, double* u, double lam, int iters, double tol,
+             double* log, hipStream_t st)
+{
+    const size_t n = (size_t)p->nvox * p->nt;
+    const unsigned g = grid64(n);
+    int rc;
+    if ((rc = launch_fwd(p, u, w.sino, st)) || (rc = launch_adj(p, w.sino, w.q, st))) return rc;
+    hipLaunchKernelGGL(k64_init, dim3(g), dim3(256), 0, st, w.q, u, b, lam, w.r, w.p, n, w.part);
+    hipLaunchKernelGGL(k64_fin_gamma, dim3(1), dim3(256), 0, st, w.part, (int)g, w.state, tol, log);
+    CHK(hipGetLastError());
+    for (int k = 0; k < iters; k++) {
+        if ((rc = launch_fwd(p, w.p, w.sino, st)) || (rc = launch_adj(p, w.sino, w.q, st))) return rc;
+        hipLaunchKernelGGL(k64_pq, dim3(g), dim3(256), 0, st, w.q, w.p, lam, n, w.part, w.state);
+        hipLaunchKernelGGL(k64_fin_alpha, dim3(1), dim3(256), 0, st, w.part, (int)g, w.state);
+        hipLaunchKernelGGL(k64_update, dim3(g), dim3(256), 0, st, u, w.r, w.p, w.q, n, w.part, w.state);
+        hipLaunchKernelGGL(k64_fin_beta, dim3(1), dim3(256), 0, st, w.part, (int)g, w.state, log);
+        hipLaunchKernelGGL(k64_p, dim3(g), dim3(256), 0, st, w.p, w.r, n, w.state);
+        CHK(hipGetLastError());
+    }
+    return FL_OK;
+}
+
+// ------------------------------------------------ FP64 framelets (level 1)
+//
+// The undecimated frame of GenerateFrameletFilter.m with circular
+// correlation (ConvSymAsym3D.m:19-24), every value in double:
+//   3-D band (i,j,k) = corr_z[D_k] corr_x[D_j] corr_y[D_i] u   (FraDecMultiLevel3D.m:35-47)
+//   2-D band (i,j)   = corr_y[D_j] corr_x[D_i] u               (FraDecMultiLevel2D.m:43-51)
+//   band b = i + NF j (+ NF^2 k); synthesis the transpose with R = flip(D)
+//   (FraRecMultiLevel{3D,2D}.m), by per-voxel gathers.
+
+struct Taps64 {
+    double D[5][5], R[5][5];
+};
+
+__device__ __forceinline__ int wrap64(int v, int n)
+{
+    v %= n;
+    return v < 0 ? v + n : v;
+}
+
+// all NB bands of voxel (x, y, z) of u (dim 3) or pixel (x, y) (dim 2)
+template <int DIM, int NF, int NT>
+__device__ __forceinline__ void bands64(const double* __restrict__ u, int nx, int ny, int nz, int x, int y, int z,
+                                        const Taps64& T, double* out)
+{
+    constexpr int C = (NT - 1) / 2;
+    if (DIM == 3) {
+        double nb[NT][NT][NT];
+        for (int kz = 0; kz < NT; kz++)
+            for (int ky = 0; ky < NT; ky++)
+                for (int kx = 0; kx < NT; kx++)
+                    nb[kz][ky][kx] = u[((size_t)wrap64(z + kz - C, nz) * ny + wrap64(y + ky - C, ny)) * nx +
+                                       wrap64(x + kx - C, nx)];
+        for (int i = 0; i < NF; i++) {
+            double ty[NT][NT];   // [kz][kx]: y pass with D_i
+            for (int kz = 0; kz < NT; kz++)
+                for (int kx = 0; kx < NT; kx++) {
+                    double s = 0.0;
+                    for (int ky = 0; ky < NT; ky++) s += T.D[i][ky] * nb[kz][ky][kx];
+                    ty[kz][kx] = s;
+                }
+            for (int j = 0; j < NF; j++) {
+                double tx[NT];   // [kz]: x pass with D_j
+                for (int kz = 0; kz < NT; kz++) {
+                    double s = 0.0;
+                    for (int kx = 0; kx < NT; kx++) s += T.D[j][kx] * ty[kz][kx];
+                    tx[kz] = s;
+                }
+                for (int k = 0; k < NF; k++) {
+                    double s = 0.0;
+                    for (int kz = 0; kz < NT; kz++) s += T.D[k][kz] * tx[kz];
+                    out[i + NF * j + NF * NF * k] = s;
+                }
+            }
+        }
+    } else {
+        double nb[NT][NT];
+        for (int ky = 0; ky < NT; ky++)
+            for (int kx = 0; kx < NT; kx++)
+                nb[ky][kx] = u[(size_t)wrap64(y + ky - C, ny) * nx + wrap64(x + kx - C, nx)];
+        for (int i = 0; i < NF; i++) {
+            double tx[NT];   // [ky]: x pass with D_i
+            for (int ky = 0; ky < NT; ky++) {
+                double s = 0.0;
+                for (int kx = 0; kx < NT; kx++) s += T.D[i][kx] * nb[ky][kx];
+                tx[ky] = s;
+            }
+            for (int j = 0; j < NF; j++) {
+                double s = 0.0;
+                for (int ky = 0; ky < NT; ky++) s += T.D[j][ky] * tx[ky];
+                out[i + NF * j] = s;
+            }
+        }
+    }
+}
+
+template <int DIM, int NF, int NT>
+__global__ __launch_bounds__(256) void k64_fwt(const double* __restrict__ u, double* __restrict__ C, int nx, int ny,
+                                              int nz, Taps64 T)
+{
+    constexpr int NB = DIM == 3 ? NF * NF * NF : NF * NF;
+    const size_t n = (size_t)nx * ny * nz;
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= n) return;
+    const int x = (int)(idx % nx), y = (int)((idx / nx) % ny), z = (int)(idx / ((size_t)nx * ny));
+    double w[NB];
+    bands64<DIM, NF, NT>(u, nx, ny, nz, x, y, z, T, w);
+    for (int b = 0; b < NB; b++) C[(size_t)b * n + idx] = w[b];
+}
+
+// out = alpha * W^T C + beta * base (base may alias out)
+template <int DIM, int NF, int NT>
+__global__ __launch_bounds__(256) void k64_iwt(const double* __restrict__ C, double* out, int nx, int ny, int nz,
+                                              Taps64 T, double alpha, double beta, const double* base)
+{
+    constexpr int CC = (NT - 1) / 2;
+    const size_t n = (size_t)nx * ny * nz;
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= n) return;
+    const int x = (int)(idx % nx), y = (int)((idx / nx) % ny), z = (int)(idx / ((size_t)nx * ny));
+    double acc = 0.0;
+    if (DIM == 3) {
+        // u = sum_i corr_y[R_i] sum_j corr_x[R_j] sum_k corr_z[R_k] C_ijk
+        for (int i = 0; i < NF; i++)
+            for (int j = 0; j < NF; j++)
+                for (int k = 0; k < NF; k++) {
+                    const double* cb = C + (size_t)(i + NF * j + NF * NF * k) * n;
+                    for (int ky = 0; ky < NT; ky++) {
+                        if (T.R[i][ky] == 0.0) continue;
+                        const int yy = wrap64(y + ky - CC, ny);
+                        for (int kx = 0; kx < NT; kx++) {
+                            if (T.R[j][kx] == 0.0) continue;
+                            const int xx = wrap64(x + kx - CC, nx);
+                            double s = 0.0;
+                            for (int kz = 0; kz < NT; kz++)
+                                s += T.R[k][kz] * cb[((size_t)wrap64(z + kz - CC, nz) * ny + yy) * nx + xx];
+                            acc += T.R[i][ky] * T.R[j][kx] * s;
+                        }
+                    }
+                }
+    } else {
+        // u = sum_i corr_x[R_i] sum_j corr_y[R_j] C_ij
+        for (int i = 0; i < NF; i++)
+            for (int j = 0; j < NF; j++) {
+                const double* cb = C + (size_t)(i + NF * j) * n;
+                for (int kx = 0; kx < NT; kx++) {
+                    if (T.R[i][kx] == 0.0) continue;
+                    const int xx = wrap64(x + kx - CC, nx);
+                    double s = 0.0;
+                    for (int ky = 0; ky < NT; ky++) s += T.R[j][ky] * cb[(size_t)wrap64(y + ky - CC, ny) * nx + xx];
+                    acc += T.R[i][kx] * s;
+                }
+            }
+    }
+    double v = alpha * acc;
+    if (base) v += beta * base[idx];
+    out[idx] = v;
+}
+
+// Split-Bregman coefficient step (solveSplitBregman.m:28-36) in double:
+// t = v + Wu, alpha = shrink(t, tau) (low-pass copied), v <- v + (Wu - alpha),
+// d = alpha - v; partials of |Wu - alpha|^2
+template <int DIM, int NF, int NT>
+__global__ __launch_bounds__(256) void k64_bregman(const double* __restrict__ u, double* __restrict__ V,
+                                                  double* __restrict__ Dc, int nx, int ny, int nz, Taps64 T, double tau,
+                                                  double* __restrict__ part)
+{
+    constexpr int NB = DIM == 3 ? NF * NF * NF : NF * NF;
+    const size_t n = (size_t)nx * ny * nz;
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    double r2 = 0.0;
+    if (idx < n) {
+        const int x = (int)(idx % nx), y = (int)((idx / nx) % ny), z = (int)(idx / ((size_t)nx * ny));
+        double w[NB];
+        bands64<DIM, NF, NT>(u, nx, ny, nz, x, y, z, T, w);
+        for (int b = 0; b < NB; b++) {
+            const size_t o = (size_t)b * n + idx;
+            const double v = V[o], t = v + w[b];
+            double a = t;
+            if (b != 0) {
+                const double m = fabs(t) - tau;
+                a = m > 0.0 ? copysign(m, t) : 0.0;
+            }
+            const double e = w[b] - a, vn = v + e;
+            r2 += e * e;
+            V[o] = vn;
+            Dc[o] = a - vn;
+        }
+    }
+    part_store(r2, part);
+}
+
+__global__ __launch_bounds__(256) void k64_sum_to(const double* __restrict__ part, int np, double* out)
+{
+    const double v = part_sum(part, np);
+    if (threadIdx.x == 0) *out = v;
+}
+
+#define FRAME64_SWITCH(dim, kind, CALL)                                                        \
+    do {                                                                                       \
+        if (dim == 3) {                                                                        \
+            constexpr int DIM = 3;                                                             \
+            if (kind == FL_FRAME_HAAR) { constexpr int NF = 2, NT = 3; CALL; }                \
+            else if (kind == FL_FRAME_LINEAR) { constexpr int NF = 3, NT = 3; CALL; }         \
+            else { constexpr int NF = 5, NT = 5; CALL; }                                       \
+        } else {                                                                               \
+            constexpr int DIM = 2;                                                             \
+            if (kind == FL_FRAME_HAAR) { constexpr int NF = 2, NT = 3; CALL; }                \
+            else if (kind == FL_FRAME_LINEAR) { constexpr int NF = 3, NT = 3; CALL; }         \
+            else { constexpr int NF = 5, NT = 5; CALL; }                                       \
+        }                                                                                      \
+    } while (0)
+
+}  // namespace
+
 extern "C" int fl_cgls_f64(fl_plan p, const double* d_b, double* d_u, double lam, int iters, double tol,
                            double* gamma_log, int* iters_done, void* stream)
 {
@@ -455,34 +694,67 @@ extern "C" int fl_cgls_f64(fl_plan p, const double* d_b, double* d_u, double lam
     hipStream_t st = (hipStream_t)stream;
     const size_t n = (size_t)p->nvox * p->nt, ns = (size_t)p->ndet * p->nv;
     DevMem m;
-    double *r, *pp, *q, *sino, *part, *log;
-    Cg64* state;
-    if ((rc = m.get(&r, n, "r")) || (rc = m.get(&pp, n, "p")) || (rc = m.get(&q, n, "q")) ||
-        (rc = m.get(&sino, ns, "sinogram")) || (rc = m.get(&part, NP64, "partials")) ||
-        (rc = m.get(&log, (size_t)iters + 1, "log")) || (rc = m.get(&state, 1, "state")))
-        return rc;
-    const unsigned g = grid64(n);
-    // r = (A^T A + lam I) u0 - b, p = -r, gamma = r.r
-    if ((rc = launch_fwd(p, d_u, sino, st)) || (rc = launch_adj(p, sino, q, st))) return rc;
-    hipLaunchKernelGGL(k64_init, dim3(g), dim3(256), 0, st, q, d_u, d_b, lam, r, pp, n, part);
-    hipLaunchKernelGGL(k64_fin_gamma, dim3(1), dim3(256), 0, st, part, (int)g, state, tol, log);
-    CHK(hipGetLastError());
-    for (int k = 0; k < iters; k++) {
-        if ((rc = launch_fwd(p, pp, sino, st)) || (rc = launch_adj(p, sino, q, st))) return rc;
-        hipLaunchKernelGGL(k64_pq, dim3(g), dim3(256), 0, st, q, pp, lam, n, part, state);
-        hipLaunchKernelGGL(k64_fin_alpha, dim3(1), dim3(256), 0, st, part, (int)g, state);
-        hipLaunchKernelGGL(k64_update, dim3(g), dim3(256), 0, st, d_u, r, pp, q, n, part, state);
-        hipLaunchKernelGGL(k64_fin_beta, dim3(1), dim3(256), 0, st, part, (int)g, state, log);
-        hipLaunchKernelGGL(k64_p, dim3(g), dim3(256), 0, st, pp, r, n, state);
-        CHK(hipGetLastError());
-    }
+    Cg64Ws w;
+    double* log;
+    if ((rc = cg64_alloc(m, w, n, ns)) || (rc = m.get(&log, (size_t)iters + 1, "log"))) return rc;
+    if ((rc = cg64_run(p, w, d_b, d_u, lam, iters, tol, log, st))) return rc;
     Cg64 s{};
-    CHK(hipMemcpyAsync(&s, state, sizeof s, hipMemcpyDeviceToHost, st));
+    CHK(hipMemcpyAsync(&s, w.state, sizeof s, hipMemcpyDeviceToHost, st));
     if (gamma_log) CHK(hipMemcpyAsync(gamma_log, log, ((size_t)iters + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
     // the workspace is freed on return: wait for the solve
     CHK(hipStreamSynchronize(st));
     if (iters_done) *iters_done = s.iter;
     if (gamma_log)
         for (int k = s.iter + 1; k <= iters; k++) gamma_log[k] = s.gamma;
+    return fl_ok();
+}
+
+extern "C" int fl_split_bregman_f64(fl_plan p, int kind, const double* d_f, double* d_u, double lam, double mu,
+                                    int outer, int inner, double* resid_log, double* d_u_hist, void* stream)
+{
+    if (!p || !d_f || !d_u || outer < 0 || inner < 0 || !(mu > 0))
+        return fl_fail(FL_ERR_INVALID, "bad FP64 split-Bregman arguments");
+    if (kind != FL_FRAME_HAAR && kind != FL_FRAME_LINEAR && kind != FL_FRAME_CUBIC)
+        return fl_fail(FL_ERR_INVALID, "bad framelet kind");
+    if (p->nt != 1) return fl_fail(FL_ERR_UNSUPPORTED, "split Bregman on one frame");
+    int rc = ensure_tables(p);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const int dim = p->cone ? 3 : 2;
+    const int nx = p->fg.nx, ny = p->fg.ny, nz = p->cone ? p->fg.nz : 1;
+    const int nf = kind == FL_FRAME_HAAR ? 2 : kind == FL_FRAME_LINEAR ? 3 : 5;
+    const int nb = dim == 3 ? nf * nf * nf : nf * nf;
+    const size_t n = (size_t)p->nvox, ns = (size_t)p->ndet * p->nv;
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    Taps64 T;
+    fl_frame_taps_f64(kind, T.D, T.R);
+    DevMem m;
+    Cg64Ws w;
+    double *V, *Dc, *Atf, *b, *fpart, *res;
+    if ((rc = cg64_alloc(m, w, n, ns)) || (rc = m.get(&V, (size_t)nb * n, "v")) ||
+        (rc = m.get(&Dc, (size_t)nb * n, "d")) || (rc = m.get(&Atf, n, "Atf")) || (rc = m.get(&b, n, "b")) ||
+        (rc = m.get(&fpart, blocks, "framelet partials")) || (rc = m.get(&res, (size_t)(outer > 0 ? outer : 1), "residuals")))
+        return rc;
+    // v = alpha = W u0 (so alpha - v = 0 and b = A^T f)
+    FRAME64_SWITCH(dim, kind, hipLaunchKernelGGL((k64_fwt<DIM, NF, NT>), dim3(blocks), dim3(256), 0, st, d_u, V, nx,
+                                                 ny, nz, T));
+    CHK(hipGetLastError());
+    if ((rc = launch_adj(p, d_f, Atf, st))) return rc;
+    CHK(hipMemcpyAsync(b, Atf, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    for (int k = 0; k < outer; k++) {
+        if ((rc = cg64_run(p, w, b, d_u, mu, inner, 0.0, nullptr, st))) return rc;
+        if (d_u_hist) CHK(hipMemcpyAsync(d_u_hist + (size_t)k * n, d_u, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        FRAME64_SWITCH(dim, kind, hipLaunchKernelGGL((k64_bregman<DIM, NF, NT>), dim3(blocks), dim3(256), 0, st, d_u,
+                                                     V, Dc, nx, ny, nz, T, lam / mu, fpart));
+        hipLaunchKernelGGL(k64_sum_to, dim3(1), dim3(256), 0, st, fpart, (int)blocks, res + k);
+        // b = A^T f + mu W^T (alpha - v)
+        FRAME64_SWITCH(dim, kind, hipLaunchKernelGGL((k64_iwt<DIM, NF, NT>), dim3(blocks), dim3(256), 0, st, Dc, b,
+                                                     nx, ny, nz, T, mu, 1.0, Atf));
+        CHK(hipGetLastError());
+    }
+    if (resid_log && outer > 0)
+        CHK(hipMemcpyAsync(resid_log, res, (size_t)outer * sizeof(double), hipMemcpyDeviceToHost, st));
+    // the workspace is freed on return: wait for the solve
+    CHK(hipStreamSynchronize(st));
     return fl_ok();
 }

@@ -16,6 +16,8 @@ struct FlFrameDesc {
 };
 
 int fl_frame_nf(int kind);
+// the filter taps in double (decomposition D, reconstruction R; NT taps of NF filters)
+void fl_frame_taps_f64(int kind, double (&D)[5][5], double (&R)[5][5]);
 unsigned fl_frame_blocks(const FlFrameDesc& f);
 hipError_t fl_launch_fwt(const FlFrameDesc& f, const void* u, int ut, void* C, int ct, int level, hipStream_t st);
 hipError_t fl_launch_iwt(const FlFrameDesc& f, const void* C, int ct, void* u, int ut, int level, float alpha,

@@ -579,20 +579,23 @@ __global__ __launch_bounds__(256) void k_l1grad2(const float* __restrict__ u, CT
 
 #include "fl_framelet.h"
 
-FlTaps fl_make_taps(int kind)
+// The filter taps of GenerateFrameletFilter.m:1-30, D[i] (decomposition)
+// and R[i] = flip(D[i]) (reconstruction), NT taps each, in double.
+void fl_frame_taps_f64(int kind, double (&Dd)[5][5], double (&Rd)[5][5])
 {
-    FlTaps T = {};
     const double s2 = 1.4142135623730951, s6 = 2.449489742783178;
+    for (int i = 0; i < 5; i++)
+        for (int k = 0; k < 5; k++) Dd[i][k] = Rd[i][k] = 0.0;
     if (kind == FL_FRAME_HAAR) {
         const double D[2][3] = {{0, 0.5, 0.5}, {0, -0.5, 0.5}};
         const double R[2][3] = {{0.5, 0.5, 0}, {0.5, -0.5, 0}};
         for (int i = 0; i < 2; i++)
-            for (int k = 0; k < 3; k++) { T.D[i][k] = (float)D[i][k]; T.R[i][k] = (float)R[i][k]; }
+            for (int k = 0; k < 3; k++) { Dd[i][k] = D[i][k]; Rd[i][k] = R[i][k]; }
     } else if (kind == FL_FRAME_LINEAR) {
         const double D[3][3] = {{0.25, 0.5, 0.25}, {-s2 / 4, 0, s2 / 4}, {-0.25, 0.5, -0.25}};
         const double R[3][3] = {{0.25, 0.5, 0.25}, {s2 / 4, 0, -s2 / 4}, {-0.25, 0.5, -0.25}};
         for (int i = 0; i < 3; i++)
-            for (int k = 0; k < 3; k++) { T.D[i][k] = (float)D[i][k]; T.R[i][k] = (float)R[i][k]; }
+            for (int k = 0; k < 3; k++) { Dd[i][k] = D[i][k]; Rd[i][k] = R[i][k]; }
     } else {
         const double D[5][5] = {{1 / 16., 4 / 16., 6 / 16., 4 / 16., 1 / 16.},
                                 {-1 / 8., -2 / 8., 0, 2 / 8., 1 / 8.},
@@ -605,8 +608,17 @@ FlTaps fl_make_taps(int kind)
                                 {1 / 8., -2 / 8., 0, 2 / 8., -1 / 8.},
                                 {-1 / 16., 4 / 16., -6 / 16., 4 / 16., -1 / 16.}};
         for (int i = 0; i < 5; i++)
-            for (int k = 0; k < 5; k++) { T.D[i][k] = (float)D[i][k]; T.R[i][k] = (float)R[i][k]; }
+            for (int k = 0; k < 5; k++) { Dd[i][k] = D[i][k]; Rd[i][k] = R[i][k]; }
     }
+}
+
+FlTaps fl_make_taps(int kind)
+{
+    FlTaps T = {};
+    double D[5][5], R[5][5];
+    fl_frame_taps_f64(kind, D, R);
+    for (int i = 0; i < 5; i++)
+        for (int k = 0; k < 5; k++) { T.D[i][k] = (float)D[i][k]; T.R[i][k] = (float)R[i][k]; }
     return T;
 }
 

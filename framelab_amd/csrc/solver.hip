@@ -687,7 +687,8 @@ static hipError_t synth_rhs(const FlFrameDesc& f, int level, int nbands, void* D
 }
 
 extern "C" int fl_split_bregman(fl_plan pl, int kind, int level, const float* d_f, float* d_u, float lam, float mu,
-                                int outer, int inner, double cg_tol, int flags, double* resid_log, void* stream)
+                                int outer, int inner, double cg_tol, int flags, double* resid_log, float* d_u_hist,
+                                void* stream)
 {
     if (!pl || !d_f || !d_u || outer < 0 || inner < 0 || !(mu > 0) || level < 1 || level > 8)
         return fl_fail(FL_ERR_INVALID, "bad split-Bregman arguments");
@@ -747,6 +748,13 @@ extern "C" int fl_split_bregman(fl_plan pl, int kind, int level, const float* d_
     HIPCHK(fl_launch_copy(w.Atf, w.b, n, st));
     for (int k = 0; k < outer; k++) {
         if ((rc = cg_run(pl, w, w.b, d_u, mu, inner, cg_tol, flags))) break;
+        if (d_u_hist) {
+            const hipError_t eh = fl_launch_copy(d_u, d_u_hist + (size_t)k * n, n, st);
+            if (eh != hipSuccess) {
+                rc = fl_hip_fail(eh, "iterate history");
+                break;
+            }
+        }
         // per level: analysis of the previous level's low-pass band of W u
         // (u itself at level 1), shrink, Bregman update
         hipError_t e = hipSuccess;

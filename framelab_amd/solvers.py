@@ -78,20 +78,40 @@ def cgls_f64(P: Projector, b: torch.Tensor, u0: torch.Tensor, lam: float = 0.0, 
     return u, np.array(glog[:]), done.value
 
 
+def split_bregman_f64(P: Projector, kind: str, f: torch.Tensor, u0: torch.Tensor, lam: float, mu: float,
+                      outer: int, inner: int, history: bool = False):
+    """Split Bregman with every value in double (FP64 projector pair, FP64
+    one-level framelets): the accuracy reference of the FP32 / BF16 / FP16
+    solves.  Returns (u, constraint residual per outer step), and with
+    history=True also the iterate after each outer step (outer volumes)."""
+    from .framelets import _KIND
+    u = u0.detach().to(torch.float64).contiguous().clone()
+    f = f.to(torch.float64).contiguous()
+    res = (C.c_double * max(outer, 1))()
+    hist = torch.empty((max(outer, 1),) + tuple(u.shape), dtype=torch.float64, device=u.device) if history else None
+    _lib.check(_lib.lib().fl_split_bregman_f64(P.handle, _KIND[kind], C.c_void_p(f.data_ptr()),
+                                               C.c_void_p(u.data_ptr()), float(lam), float(mu), int(outer),
+                                               int(inner), res, C.c_void_p(hist.data_ptr() if history else 0),
+                                               _stream()))
+    return (u, np.array(res[:outer]), hist[:outer]) if history else (u, np.array(res[:outer]))
+
+
 def split_bregman(P: Projector, kind: str, f: torch.Tensor, u0: torch.Tensor, lam: float, mu: float,
                   outer: int, inner: int, cg_tol: float = 0.0, graph: bool = False, matched: bool = False,
-                  storage=torch.float32, level: int = 1):
+                  storage=torch.float32, level: int = 1, history: bool = False):
     """Device-resident split Bregman with a `level`-level frame.  Returns (u,
-    constraint residual per outer step)."""
+    constraint residual per outer step), and with history=True also the
+    iterate after each outer step (outer volumes)."""
     from .framelets import _KIND
     u = u0.detach().float().contiguous().clone()
     f = f.float().contiguous()
     res = (C.c_double * max(outer, 1))()
+    hist = torch.empty((max(outer, 1),) + tuple(u.shape), dtype=torch.float32, device=u.device) if history else None
     _lib.check(_lib.lib().fl_split_bregman(P.handle, _KIND[kind], int(level), C.c_void_p(f.data_ptr()),
                                            C.c_void_p(u.data_ptr()), float(lam), float(mu), int(outer),
                                            int(inner), float(cg_tol), _flags(graph, matched, storage), res,
-                                           _stream()))
-    return u, np.array(res[:outer])
+                                           C.c_void_p(hist.data_ptr() if history else 0), _stream()))
+    return (u, np.array(res[:outer]), hist[:outer]) if history else (u, np.array(res[:outer]))
 
 
 class CGLS:

@@ -203,10 +203,11 @@ int fl_cgls(fl_plan plan, const float *d_b, float *d_u, float lam, int iters, do
  * level frame of FrameletSystem(dim, kind, level) (FrameletSystem.m:18-27).
  * d_f: sinogram; d_u: u0 in, u out.  resid_log (outer doubles, optional):
  * |Wu - alpha|^2 over all levels after each outer step (requesting it
- * synchronises). */
+ * synchronises).  d_u_hist (device, outer volumes, optional) receives the
+ * iterate after each outer step's CGLS. */
 int fl_split_bregman(fl_plan plan, int frame_kind, int level, const float *d_f, float *d_u,
                      float lam, float mu, int outer, int inner, double cg_tol, int flags,
-                     double *resid_log, void *stream);
+                     double *resid_log, float *d_u_hist, void *stream);
 
 /* CGLS over caller-supplied operators (the class mirror's CGLS.solveCGLS
  * with arbitrary A / At): the vector steps of solveCGLS.m:7-30 on FP32
@@ -342,6 +343,14 @@ int fl_forward_f64(fl_plan plan, const double *d_x, double *d_y, void *stream);
 int fl_adjoint_f64(fl_plan plan, const double *d_y, double *d_x, void *stream);
 int fl_cgls_f64(fl_plan plan, const double *d_b, double *d_u, double lam, int iters, double tol,
                 double *gamma_log, int *iters_done, void *stream);
+/* Split Bregman (solveSplitBregman.m:16-41, as fl_split_bregman) over the
+ * FP64 pair with a one-level framelet analysis / synthesis in double:
+ * d_f and d_u (u0 in, u out) are FP64; resid_log and d_u_hist as
+ * fl_split_bregman's.  Allocates its workspace per call and synchronises
+ * the stream. */
+int fl_split_bregman_f64(fl_plan plan, int frame_kind, const double *d_f, double *d_u, double lam,
+                         double mu, int outer, int inner, double *resid_log, double *d_u_hist,
+                         void *stream);
 
 /* Traversal log of selected rays (testing): rays = (view, ia, ib) triples
  * (host), cap entries per ray; J/S/cnt host outputs of nrays*cap / nrays. */
