@@ -304,16 +304,27 @@ def cgls_leg(P, x, iters):
     return out
 
 
-def precision_leg(P, x, outer=30, inner=3):
+def precision_leg(P, x, outer=30, inner=3, budget_s=90.0):
     """Split Bregman with the 27-band 3-D linear frame (mu 1, lam 1e-3,
     `outer` x `inner`, graph-replayed CGLS with the matched adjoint, FP64
     reductions) with the projector input and sinogram stored in BF16, in
     FP16 and in FP32, each iterate against the FP64 solve's (FP64 pair and
     FP64 framelets) after every outer step, on the 256^3 / 360-view
-    geometry (an FP64 solve at 512^3 / 720 views takes minutes)."""
+    geometry (an FP64 solve at 512^3 / 720 views takes minutes).  The FP64
+    solve runs as one call; `outer` is cut so that its estimate from one
+    timed FP64 projector pair stays within `budget_s`."""
     import torch
     from framelab_amd.solvers import split_bregman, split_bregman_f64
+    t0 = time.perf_counter()
     f64 = P.forward_f64(x.double())
+    P.adjoint_f64(f64)
+    torch.cuda.synchronize()
+    pair = time.perf_counter() - t0
+    per_outer = (inner + 1) * pair
+    fit = max(1, min(outer, int(budget_s / per_outer)))
+    log(f"bench: FP64 projector pair {pair * 1e3:.0f} ms; FP64 split Bregman {fit} outer x {inner} inner "
+        f"(about {fit * per_outer:.0f} s)")
+    outer = fit
     f32 = f64.float()
     u0 = torch.zeros_like(x)
     out = {"workload": "cone 256^3, 360 views, 512x384; split Bregman, 27-band linear frame, mu=1, lam=1e-3, "
@@ -322,6 +333,8 @@ def precision_leg(P, x, outer=30, inner=3):
     _, _, ref = split_bregman_f64(P, "linear", f64, u0.double(), 1e-3, 1.0, outer, inner, history=True)
     torch.cuda.synchronize()
     out["fp64_seconds"] = time.perf_counter() - t0
+    out["fp64_pair_seconds"] = pair
+    log(f"bench: FP64 split Bregman {out['fp64_seconds']:.1f} s")
     rnorm = ref.flatten(1).norm(dim=1)
     for name, dt in (("fp32", torch.float32), ("bf16", torch.bfloat16), ("fp16", torch.float16)):
         t0 = time.perf_counter()
