@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/framelab_hip.h"
@@ -360,6 +361,15 @@ PlanWs* get_ws(fl_plan_s* p)
         if (e_ != hipSuccess) return fl_hip_fail(e_, #expr);  \
     } while (0)
 
+// The low-precision copy of p a solve in storage type LT maintains: none in
+// FP32 (the buffer a low-precision solve on the same plan left holds two
+// bytes per element, too small for floats).
+template <typename LT>
+LT* lowp_copy(Workspace& w)
+{
+    return std::is_same<LT, float>::value ? nullptr : (LT*)w.plo;
+}
+
 // q = A^T (A x) for x the FP32 vector (or its low-precision copy)
 int normal_op(fl_plan_s* pl, Workspace& w, const float* x, const void* xlo, int flags, float* out, hipStream_t st)
 {
@@ -385,7 +395,7 @@ int cg_body_t(fl_plan_s* pl, Workspace& w, float* u, float lam, int flags, hipSt
     hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(256), 0, st, w.part, (int)g, w.st);
     hipLaunchKernelGGL(k_cg_update, dim3(g), dim3(256), 0, st, u, w.r, w.p, w.q, n, w.part, w.st);
     hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(256), 0, st, w.part, (int)g, w.st, w.log, w.maxlog);
-    hipLaunchKernelGGL((k_cg_p<LT>), dim3(g), dim3(256), 0, st, w.p, w.r, (LT*)w.plo, n, w.st);
+    hipLaunchKernelGGL((k_cg_p<LT>), dim3(g), dim3(256), 0, st, w.p, w.r, lowp_copy<LT>(w), n, w.st);
     HIPCHK(hipGetLastError());
     return FL_OK;
 }
@@ -410,11 +420,12 @@ int cg_init_t(fl_plan_s* pl, Workspace& w, const float* b, float* u, float lam, 
     if (lowp_type(flags) == FL_F32) {
         rc = normal_op(pl, w, u, nullptr, flags, w.q, st);
     } else {
-        hipLaunchKernelGGL((k_cast<LT>), dim3(g), dim3(256), 0, st, u, (LT*)w.plo, n);
+        hipLaunchKernelGGL((k_cast<LT>), dim3(g), dim3(256), 0, st, u, lowp_copy<LT>(w), n);
         rc = normal_op(pl, w, nullptr, w.plo, flags, w.q, st);
     }
     if (rc) return rc;
-    hipLaunchKernelGGL((k_cg_init<LT>), dim3(g), dim3(256), 0, st, w.q, u, b, lam, w.r, w.p, (LT*)w.plo, n, w.part);
+    hipLaunchKernelGGL((k_cg_init<LT>), dim3(g), dim3(256), 0, st, w.q, u, b, lam, w.r, w.p, lowp_copy<LT>(w), n,
+                       w.part);
     hipLaunchKernelGGL(k_fin_gamma, dim3(1), dim3(256), 0, st, w.part, (int)g, w.st, w.log);
     HIPCHK(hipGetLastError());
     return FL_OK;

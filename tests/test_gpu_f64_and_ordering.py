@@ -305,7 +305,7 @@ def test_forward_norm_checks_and_dropin_ynorm(gpu, oracle):
     SO, OD = float(p.SO), float(p.OD)
     dx = (c[:, None, None] * OD - s[:, None, None] * ya[None, None, :]) + SO * c[:, None, None]
     dy = (s[:, None, None] * OD + c[:, None, None] * ya[None, None, :]) + SO * s[:, None, None]
-    dz = np.broadcast_to(zb[None, :, None], dx.shape[:2] + (1,))
+    dz = zb[None, :, None]
     L64 = np.sqrt(dx ** 2 + dy ** 2 + dz ** 2)
     want = y.reshape(L64.shape) / (L64 * float(p.scale))
     rel = np.abs(yn.reshape(L64.shape) - want).max() / np.abs(want).max()

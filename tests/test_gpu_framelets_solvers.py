@@ -166,6 +166,29 @@ def test_cgls_low_precision_storage(gpu, oracle):
         assert rel < tol
 
 
+def test_fp32_solve_after_low_precision_solve(gpu):
+    """An FP32-storage solve on a plan whose workspace an FP16 solve has
+    already set up (its two-byte copy of p exists) equals, bit for bit, the
+    same solve on a fresh plan: eager and graph-replayed, exact and matched
+    adjoint.  Regression test: the FP32 iteration once wrote four-byte
+    values into that two-byte buffer."""
+    _say("case: FP32 CGLS after an FP16 CGLS on one plan")
+    from framelab_amd.phantoms import ellipsoids_3d
+    from framelab_amd.projector import Projector
+    from framelab_amd.solvers import cgls
+    p = small_cone(na=48, nb=40, nv=20, N=24)
+    x = torch.from_numpy(ellipsoids_3d(24, 6, seed=4)).to(gpu)
+    P, Q = Projector(p), Projector(p)
+    b = P.adjoint(P.forward(x))
+    u0 = torch.zeros(P.vol_shape, device=gpu)
+    cgls(P, b, u0, 0.0, 3, storage=torch.float16, graph=True)
+    for graph in (False, True):
+        for matched in (False, True):
+            u, g, _ = cgls(P, b, u0, 0.0, 5, graph=graph, matched=matched)
+            w, h, _ = cgls(Q, b, u0, 0.0, 5, graph=graph, matched=matched)
+            assert torch.equal(u, w) and np.array_equal(g, h), (graph, matched)
+
+
 def test_cgls_graph_reuse_across_calls(gpu):
     """A graph captured by a short solve is reused by a longer one (the log
     buffer grows in between) and by a solve on another iterate tensor: every
