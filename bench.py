@@ -216,6 +216,36 @@ def measured_traffic(kernel):
     return None
 
 
+VALU_PEAK_FP32, VALU_PEAK_INT = 0.239, 0.469   # wave-level VALU instructions per SIMD per cycle (measured)
+SIMDS, CLOCK_HZ = 1024, 2.4e9
+
+
+def valu_issue(kernel, ms):
+    """VALU issue rate of `kernel` over a live-timed launch of `ms`
+    milliseconds: its wave-level VALU instruction count per launch from the
+    committed counter summary (SQ_INSTS_VALU), divided by SIMDs x cycles.
+    No instruction mix issues faster than the integer peak, so the fraction
+    of that peak is a lower bound on the share of VALU issue capacity used.
+    None when the kernel has no entry."""
+    path = os.path.join(ROOT, "profiles", "cone256_valu.tsv")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("#"):
+                    continue
+                name, insts, lanes = line.rstrip("\n").split("\t")
+                if name == kernel:
+                    rate = float(insts) / (SIMDS * CLOCK_HZ * ms * 1e-3)
+                    return {"bound": "valu-issue", "kernel": kernel, "instructions_per_launch": float(insts),
+                            "achieved": rate, "unit": "VALU instructions per SIMD per cycle",
+                            "peak_int": VALU_PEAK_INT, "peak_fp32": VALU_PEAK_FP32,
+                            "frac": rate / VALU_PEAK_INT, "lane_efficiency": float(lanes),
+                            "source": "profiles/cone256_valu_pmc.md (rocprofv3 --pmc SQ_INSTS_VALU ...)"}
+    except OSError:
+        pass
+    return None
+
+
 def time_cgls(P, b, u0, iters, **kw):
     """Seconds per CGLS iteration (the init's normal-operator pass excluded)."""
     import torch
@@ -647,6 +677,8 @@ def main():
             "algorithmic_bytes_per_launch": B_op, "kernel_ms": dom_ms,
             "traffic_source": "profiles/cone256_hbm_traffic_pmc.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"}
 
+    valu = valu_issue(dom_name, dom_ms)
+
     # matched adjoint and low-precision storage variants (reported beside)
     extra = {"t_A_ms": mA, "t_At_exact_ms": mAt}
     evm = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
@@ -684,6 +716,7 @@ def main():
                    "volume": [N, N, N], "views": nv, "detector": [na, nb],
                    "phantom": "20 seeded ellipsoids (seed 2)"},
         "roofline": roof,
+        "valu": valu,
         "extra": extra,
         "framelet": framelet,
     }
