@@ -64,7 +64,9 @@ def cpu_info():
 
 
 def _median_s(fn, runs=5):
+    """Median wall time of `runs` calls after one untimed warm-up call."""
     import statistics
+    fn()
     ts = []
     for _ in range(runs):
         t0 = time.perf_counter()
@@ -83,14 +85,14 @@ def _calibration():
         return None
 
 
-def cpu_baseline(p, X, views=5, adj_views=3):
+def cpu_baseline(p, X, views=5, adj_views=5):
     """The oracle's C restatement of the reference's CPU path (the reference
     code does not travel to this host) on a bounded sample of the workload:
     one Siddon forward (median over `views` distinct views) plus one exact
     adjoint (median over the first `adj_views` of them) of the full 256^3
-    volume for one view, on 1 core (the reference is single-threaded) and on
-    every core of this process (OpenMP over rays / voxels): about 35 s of
-    single-core work.  The restatement follows the reference's float
+    volume for one view, after one untimed warm-up of each, on 1 core (the
+    reference is single-threaded) and on every core of this process (OpenMP
+    over rays / voxels): about 35 s of single-core work.  The restatement follows the reference's float
     operations, and the reference / restatement ratio measured on one core
     in the build container rescales the 1-core figure to the reference's."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -110,6 +112,7 @@ def cpu_baseline(p, X, views=5, adj_views=3):
     out, times = {}, {}
     for threads in (1, ncpu):
         O.set_threads(threads)
+        O.atx_cone_f32(O.siddon_cone_f32(X, subs[0]), subs[0])   # warm-up
         tA, tAt = [], []
         for k, sub in enumerate(subs):
             t0 = time.perf_counter()
@@ -123,7 +126,7 @@ def cpu_baseline(p, X, views=5, adj_views=3):
         times[threads] = (a, at)
         out[threads] = {"value": 2.0 * nvox / (a + at) / 1e9, "unit": "GUPS", "cores": threads, "kind": "port",
                         "sample": f"one forward + one exact adjoint of the full 256^3 volume for one view of 512x384 "
-                                  f"cells (forward: median over {views} views, t_A={a:.2f}s; adjoint: median over "
+                                  f"cells after one untimed warm-up (forward: median over {views} views, t_A={a:.2f}s; adjoint: median over "
                                   f"{min(views, adj_views)}, t_At={at:.2f}s); host: {model}, {cores} logical CPUs"}
     O.set_threads(1)
     res = dict(out[1])
@@ -143,9 +146,10 @@ def cpu_baseline(p, X, views=5, adj_views=3):
 def cpu_c1(dev):
     """The 2-D fan configuration timed in full on the CPU restatement: 256^2
     image of 10 seeded ellipses, 360 views x 512 bins, SO 4, SD 8; Siddon
-    forward A, exact pixel-driven adjoint A^T (medians of 5), and 20 CGLS
-    iterations on the noiseless sinogram, on 1 core and on every core; the
-    same workload on the GPU beside it."""
+    forward A, exact pixel-driven adjoint A^T (medians of 5 after a warm-up),
+    and 20 CGLS iterations on the noiseless sinogram, on 1 core and on every
+    core; the same workload on the GPU beside it (A and A^T medians of 20
+    after a warm-up)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import torch
@@ -157,7 +161,7 @@ def cpu_c1(dev):
     p = G.fan_para(512, 360, N=(256, 256), SO=4.0, OD=4.0, Ly=G.full_fan_width())
     X = ellipses_2d(256, 10, seed=1).astype(np.float32)
     ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    out = {"workload": "fan 256^2, 360 views x 512 bins, SO=4 SD=8: A, A^T (medians of 5) and 20 CGLS iterations"}
+    out = {"workload": "fan 256^2, 360 views x 512 bins, SO=4 SD=8: A, A^T (CPU medians of 5, GPU medians of 20, each after a warm-up) and 20 CGLS iterations"}
     y = O.siddon_fan_f32(X, p)
     for threads in (1, ncpu):
         O.set_threads(threads)
@@ -177,7 +181,7 @@ def cpu_c1(dev):
     P.adjoint(yg)
     evs = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
     tg = []
-    for _ in range(5):
+    for _ in range(20):
         evs[0].record()
         P.forward(x, out=yg)
         evs[1].record()
@@ -576,8 +580,8 @@ def c4_leg():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cgls-iters", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-cgls", action="store_true", help="skip the CGLS leg")
