@@ -76,16 +76,26 @@ struct Bricks {
         : nbx((nx + BX - 1) / BX), nby((ny + BY - 1) / BY), row((unsigned)nbx * BX * BY),
           plane(row * (unsigned)nby), frame((size_t)plane * nz) {}
     // (tx, ty, tz) in the volume, so non-negative: unsigned shifts and masks,
-    // and full-rate 24-bit multiplies while a plane holds under 2^24 elements
+    // and full-rate 24-bit multiplies -- bricked layouts are launched only
+    // where every factor fits 24 bits (bricked_fits_u24), so no branch here
+    // (a branch on the plane size made the compiler fall back to the
+    // quarter-rate 32-bit multiply on every step of the walk)
     __device__ __forceinline__ unsigned at(int tx, int ty, int tz) const
     {
         const unsigned x = (unsigned)tx, y = (unsigned)ty, z = (unsigned)tz;
         if (BX == 1 && BY == 1) return (z * nby + y) * nbx + x;
         const unsigned in = (x / BX) * (BX * BY) + (y % BY) * BX + x % BX;
-        if (plane < (1u << 24)) return __umul24(z, plane) + __umul24(y / BY, row) + in;
-        return z * plane + (y / BY) * row + in;
+        return __umul24(z, plane) + __umul24(y / BY, row) + in;
     }
 };
+
+// The bricked layouts' index arithmetic needs a brick plane (8 x 8 bricks,
+// the largest) of under 2^24 elements and under 2^24 planes.
+static bool bricked_fits_u24(int nx, int ny, int nz)
+{
+    const Bricks<8, 8> br(nx, ny, nz);
+    return br.plane < (1u << 24) && nz < (1 << 24);
+}
 
 template <typename VT, int BX, int BY>
 __global__ __launch_bounds__(256) void k_to_bricks(const VT* __restrict__ x, VT* __restrict__ xb, int nx, int ny,
@@ -208,7 +218,10 @@ template <typename VT, typename YT>
 static hipError_t launch_cone_t(const void* X, void* Y, const FwdGeom& g, hipStream_t st)
 {
     if (g.tile != 0 && g.tile != 3 && !g.bricks) return hipErrorInvalidValue;
-    switch (g.tile) {
+    // planes too large for the bricks' 24-bit index arithmetic take the
+    // plain layout of the same workgroup shape family
+    const int tile = (g.tile == 0 || g.tile == 3 || bricked_fits_u24(g.nx, g.ny, g.nz)) ? g.tile : 3;
+    switch (tile) {
     case 0: return launch_cone_shape<VT, YT, 64, 4, 64, 1, 1>(X, Y, g, st);
     case 1: return launch_cone_shape<VT, YT, 16, 16, 8, 8, 8>(X, Y, g, st);
     case 2: return launch_cone_shape<VT, YT, 32, 8, 8, 8, 8>(X, Y, g, st);
