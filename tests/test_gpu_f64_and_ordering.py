@@ -352,3 +352,31 @@ def test_split_bregman_f64_vs_oracle(gpu, oracle, geom):
     _say(f"  rel {rel:.2e}, residuals {res}")
     assert rel <= 1e-9
     assert np.all(np.isfinite(res)) and res[-1] < res[0]
+
+
+def test_split_bregman_iterate_history(gpu):
+    """The per-outer-step iterate history of both split-Bregman entry points:
+    entry k equals, bit for bit, a fresh solve with k+1 outer steps, and the
+    last entry equals the returned iterate (FP32 graph-replayed with the
+    matched adjoint and FP16 storage, and FP64)."""
+    from framelab_amd.phantoms import ellipses_2d
+    from framelab_amd.projector import Projector
+    from framelab_amd.solvers import split_bregman, split_bregman_f64
+    _say("case: split Bregman iterate history")
+    p = small_fan(nd=96, nv=60, N=48)
+    P = Projector(p)
+    x = torch.from_numpy(ellipses_2d(48, 6, seed=4).astype(np.float32)).to(gpu)
+    f = P.forward(x)
+    u0 = torch.zeros(P.vol_shape, device=gpu)
+    outer, inner, lam, mu = 4, 3, 1e-2, 1.0
+    kw = dict(graph=True, matched=True, storage=torch.float16)
+    u, res, hist = split_bregman(P, "linear", f, u0, lam, mu, outer, inner, history=True, **kw)
+    assert hist.shape == (outer,) + tuple(P.vol_shape)
+    assert torch.equal(hist[-1], u)
+    for k in range(outer - 1):
+        uk, _ = split_bregman(P, "linear", f, u0, lam, mu, k + 1, inner, **kw)
+        assert torch.equal(hist[k], uk), k
+    u64, res64, h64 = split_bregman_f64(P, "linear", f.double(), u0.double(), lam, mu, outer, inner, history=True)
+    assert torch.equal(h64[-1], u64)
+    uk64, _ = split_bregman_f64(P, "linear", f.double(), u0.double(), lam, mu, 2, inner)
+    assert torch.equal(h64[1], uk64)
