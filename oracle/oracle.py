@@ -19,7 +19,9 @@ reference (no reference values exist for them: MATLAB is absent and no
 reference test holds any): the framelet band values (fradec*/frarec*; pinned
 only by scipy correlation, W^T W = I and the tight-frame identities), the
 CGLS iterates (cgls; pinned by the dense normal-equation solution) and split
-Bregman (split_bregman; pinned by a fixed point and objective decrease).
+Bregman (split_bregman; pinned by objective decrease and by convergence to
+the minimiser of its objective, which the tests compute independently from
+the dual box-constrained least-squares problem).
 """
 from __future__ import annotations
 
@@ -578,7 +580,9 @@ def split_bregman(A, At, W, Wt, lam, mu, f, u0, outer, cg_iters, cg_tol=0.0, shr
     W maps a volume to a coefficient array, Wt back; shrink(c, tau) applies
     the soft threshold to everything but the low-pass band.
     Parity unpinned against the reference (its MATLAB cannot run); pinned by
-    the fixed-point and objective-decrease checks in the tests."""
+    the objective-decrease check and by convergence to the objective's
+    minimiser, computed independently (dual bounded least squares) in the
+    tests, at one and two frame levels."""
     alpha = W(u0)
     v = alpha.copy()
     Atf = At(f)
