@@ -523,3 +523,24 @@ def test_chord_sqrt_exhaustive(gpu):
                                               C.c_void_p(torch.cuda.current_stream().cuda_stream)),
                "fl_check_chord_sqrt")
     assert int(bad.item()) == 0
+
+
+def test_zero_volume_helix(gpu):
+    """test/testZeros.m:1-17: a helical scan (16 x 16 cells, zmax 2, 2 turns
+    per unit table travel, fps 10: 21 views) of a 128^3 volume of zeros is
+    zero everywhere, and so are the adjoints of a zero sinogram (exact,
+    matched, FP64), in every storage type."""
+    from framelab_amd import geometry as G
+    from framelab_amd.projector import ADJ_EXACT, ADJ_MATCHED, Projector
+    _say("case: zeros through a helical scan")
+    p = G.cone_para(16, 16, 0, N=(128, 128, 128), kind="helix", zmax=2.0, rps=2.0, vtab=1.0, fps=10.0)
+    assert p.nv == 21 and np.ptp(np.asarray(p.sd_z)) > 0
+    P = Projector(p)
+    for dt in (torch.float32, torch.bfloat16, torch.float16):
+        y = P.forward(torch.zeros(P.vol_shape, dtype=dt, device=gpu))
+        assert int(torch.count_nonzero(y)) == 0
+        for kind in (ADJ_EXACT, ADJ_MATCHED):
+            x = P.adjoint(torch.zeros(P.sino_shape, dtype=dt, device=gpu), kind=kind)
+            assert int(torch.count_nonzero(x)) == 0
+    assert int(torch.count_nonzero(P.forward_f64(torch.zeros(P.vol_shape, dtype=torch.float64, device=gpu)))) == 0
+    assert int(torch.count_nonzero(P.adjoint_f64(torch.zeros(P.sino_shape, dtype=torch.float64, device=gpu)))) == 0
