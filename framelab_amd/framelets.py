@@ -109,6 +109,10 @@ class FrameletExpansion:
 
     __mul__ = __rmul__
 
+    def adjointFrameletTransform(self) -> torch.Tensor:
+        """u = W^T alpha with this expansion's own system (FrameletExpansion.m:36-41)."""
+        return self.system.adjointTransform(self)
+
     def softThreshold(self, tau: float) -> "FrameletExpansion":
         """sign(a)*max(|a|-tau,0) on every band except each level's low-pass
         (native kernel, in place; also returned)."""

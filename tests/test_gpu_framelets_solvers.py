@@ -467,3 +467,33 @@ def test_scanner_classes(gpu, oracle):
     y2 = fb.apply(o2)
     assert np.array_equal(y2.dataArray.cpu().numpy().ravel(),
                           oracle.siddon_fan_f32(o2.dataArray.cpu().numpy(), fb.para))
+
+
+def test_data_type_framelet_methods(gpu, oracle):
+    """ObjectData.frameletTransform / adjointFrameletTransform,
+    FrameletExpansion.adjointFrameletTransform and CTData.frameletTransform
+    (ObjectData.m:58-69, FrameletExpansion.m:36-41, CTData.m:26-30): the
+    bands equal the oracle's analysis and the synthesis restores the array
+    (W^T W = I)."""
+    _say("case: data-type framelet methods")
+    from framelab_amd.datatypes import CTData, ObjectData
+    from framelab_amd.framelets import FrameletSystem
+    g = torch.Generator(device="cpu").manual_seed(21)
+    x = torch.randn((10, 12, 14), generator=g)
+    W = FrameletSystem(3, "linear", 2)
+    obj = ObjectData(3, x.to(gpu), (2.0, 2.0, 2.0))
+    alpha = obj.frameletTransform(W)
+    D, _ = oracle.framelet_filters("linear")
+    ref = oracle.fradec3d(x.double().numpy(), D, 2)
+    nf = len(D)
+    idx = [(i, j, k) for k in range(nf) for j in range(nf) for i in range(nf)]
+    for L in range(2):
+        assert _rel(alpha.frameletArray[L].cpu().numpy(), np.stack([ref[L][t] for t in idx])) < 2e-6
+    back = alpha.adjointFrameletTransform()
+    assert _rel(back.cpu().numpy(), x.numpy()) < 2e-6
+    obj2 = ObjectData(3, torch.zeros_like(obj.dataArray), obj.L)
+    obj2.adjointFrameletTransform(W, alpha)
+    assert torch.equal(obj2.dataArray, back)
+    y = torch.randn((6, 8, 9), generator=g).to(gpu)
+    ct = CTData(None, y, None, None)
+    assert torch.equal(ct.frameletTransform(W).frameletArray, W.forwardTransform(y).frameletArray)
