@@ -21,7 +21,7 @@ void fl_frame_taps_f64(int kind, double (&D)[5][5], double (&R)[5][5]);
 unsigned fl_frame_blocks(const FlFrameDesc& f);
 hipError_t fl_launch_fwt(const FlFrameDesc& f, const void* u, int ut, void* C, int ct, int level, hipStream_t st);
 hipError_t fl_launch_iwt(const FlFrameDesc& f, const void* C, int ct, void* u, int ut, int level, float alpha,
-                         float beta, const float* base, hipStream_t st);
+                         float beta, const float* base, hipStream_t st, const void* lp = nullptr);
 // one level (dilation 2^(level-1)); lp (may be null): that level's low-pass band of W u
 hipError_t fl_launch_bregman(const FlFrameDesc& f, const float* u, void* V, void* D, int ct, float tau, int level,
                              float* lp, double* part, hipStream_t st);

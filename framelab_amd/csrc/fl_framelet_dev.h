@@ -65,10 +65,11 @@ __device__ __forceinline__ void fl_bands2(const FlTaps& T, const float (&nb)[NT]
 }
 
 // Synthesis at one pixel: sum_i corr_x[R_i] sum_j corr_y[R_j] C_ij
-// (FraRecMultiLevel2D.m:41-49); band b of C starts at C + b*band_stride.
+// (FraRecMultiLevel2D.m:41-49); band b of C starts at C + b*band_stride,
+// band 0 at lp instead where lp is given.
 template <int NF, int NT, typename CT>
 __device__ __forceinline__ float fl_synth2(const CT* __restrict__ C, size_t band_stride, int nx, int ny, int x, int y,
-                                           int step, const FlTaps& T)
+                                           int step, const FlTaps& T, const CT* lp = nullptr)
 {
     constexpr int CC = (NT - 1) / 2;
     int xs[NT], ys[NT];
@@ -80,7 +81,7 @@ __device__ __forceinline__ float fl_synth2(const CT* __restrict__ C, size_t band
     float acc = 0.0f;
     for (int i = 0; i < NF; i++)
         for (int j = 0; j < NF; j++) {
-            const CT* __restrict__ cb = C + (size_t)(i + NF * j) * band_stride;
+            const CT* __restrict__ cb = (lp && i + NF * j == 0) ? lp : C + (size_t)(i + NF * j) * band_stride;
             float sx = 0.0f;
 #pragma unroll
             for (int kx = 0; kx < NT; kx++) {

@@ -17,19 +17,39 @@
 // One thread per voxel; analysis gathers the NT^d neighbourhood once and
 // produces all NF^d bands from it (separably, in registers), so the input is
 // read once and every band is written once (the transform is write-bound).
-// 3D synthesis runs separably on LDS tiles (k_iwt3z for dilations 1 and 2,
-// k_iwt3s beyond); 2D synthesis, and 3D levels whose halo outgrows the
-// tiles, gather each band's neighbourhood per output.  All accumulate in
+// 3D synthesis runs separably on LDS tiles (k_iwt3m, marching through z,
+// for dilations 1 and 2, k_iwt3s beyond); 2D synthesis, and 3D levels whose
+// halo outgrows the tiles, gather each band's neighbourhood per output.  All accumulate in
 // FP32.
 #include <algorithm>
+#include <type_traits>
 
 #include "fl_common.h"
+#include "fl_framelet.h"
 #include "fl_framelet_dev.h"
+
+// The z-marching synthesis' target of resident waves and its use of
+// full-row tiles (build-time tuning knobs).
+#ifndef FL_IWT_MIN_WAVES
+#define FL_IWT_MIN_WAVES 2048
+#endif
+#ifndef FL_IWT_FULL_ROWS
+#define FL_IWT_FULL_ROWS 1
+#endif
 
 // ---------------------------------------------------------------- 3D ------
 
 // Coordinate v in [-n, 2n) wrapped into [0, n) without a divide.
 __device__ __forceinline__ int wrap_near(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+// Band b of a coefficient array with `nvox` elements per band; lp (may be
+// null) stands in for band 0, the low-pass band (multi-level synthesis puts
+// the reconstruction of the level above there, FraRecMultiLevel3D.m:14-17).
+template <typename CT>
+__device__ __forceinline__ const CT* band_ptr(const CT* C, int b, size_t nvox, const CT* lp)
+{
+    return (b == 0 && lp) ? lp : C + (size_t)b * nvox;
+}
 
 // NEAR: the taps reach at most one period beyond the volume (C*step <= n
 // on every axis), so wrap_near serves.
@@ -123,7 +143,7 @@ __global__ __launch_bounds__(256) void k_fwt3(const XT* __restrict__ u, CT* __re
 template <int NF, int NT, typename CT, typename XT>
 __global__ __launch_bounds__(256) void k_iwt3(const CT* __restrict__ C, XT* u, int nx, int ny,
                                              int nz, int step, FlTaps T, float alpha, float beta,
-                                             const float* base)
+                                             const float* base, const CT* lp)
 {
     constexpr int CC = (NT - 1) / 2;
     const size_t nvox = (size_t)nx * ny * nz;
@@ -150,7 +170,7 @@ __global__ __launch_bounds__(256) void k_iwt3(const CT* __restrict__ C, XT* u, i
     for (int i = 0; i < NF; i++)
         for (int j = 0; j < NF; j++)
             for (int k = 0; k < NF; k++) {
-                const CT* __restrict__ cb = C + (size_t)(i + NF * j + NF * NF * k) * nvox;
+                const CT* __restrict__ cb = band_ptr(C, i + NF * j + NF * NF * k, nvox, lp);
                 // innermost corr_z[R_k], then corr_x[R_j], then corr_y[R_i]
                 float sy = 0.0f;
 #pragma unroll
@@ -195,7 +215,8 @@ __host__ __device__ inline size_t iw_lds_bytes(int h)
 
 template <int NF, int NT, typename CT, typename XT>
 __global__ __launch_bounds__(256) void k_iwt3s(const CT* __restrict__ C, XT* u, int nx, int ny, int nz, int step,
-                                              int zc, FlTaps T, float alpha, float beta, const float* base)
+                                              int zc, FlTaps T, float alpha, float beta, const float* base,
+                                              const CT* lp)
 {
     extern __shared__ float iw_lds[];
     constexpr int CC = (NT - 1) / 2;
@@ -226,7 +247,7 @@ __global__ __launch_bounds__(256) void k_iwt3s(const CT* __restrict__ C, XT* u, 
                     float t = 0.0f;
 #pragma unroll
                     for (int k = 0; k < NF; k++) {
-                        const CT* __restrict__ cb = C + (size_t)(i + NF * j + NF * NF * k) * nvox + off;
+                        const CT* __restrict__ cb = band_ptr(C, i + NF * j + NF * NF * k, nvox, lp) + off;
 #pragma unroll
                         for (int kz = 0; kz < NT; kz++) t = __builtin_fmaf(T.R[k][kz], fl_load(cb, zoff[kz]), t);
                     }
@@ -267,121 +288,192 @@ __global__ __launch_bounds__(256) void k_iwt3s(const CT* __restrict__ C, XT* u, 
     }
 }
 
-// The same separable synthesis with the z reuse kept in registers, for
-// dilations 1 and 2 (compile-time STEP, so the z window is indexed
-// statically).  A workgroup owns the 64x16 tile for ZC output slices and
-// walks the band pairs (i, j): each thread loads, for its halo positions,
-// the NF bands (i, j, *) on all ZC + 2H input slices once and forms the
-// z pass of every output slice from registers (LDS, per slice); the x pass
-// is accumulated over j per slice (LDS), and after the last j the y pass
-// goes into the outputs.  Each coefficient is read (ZC + 2H) / ZC
-// times instead of NT times, without depending on L2 to keep the slices
-// between the passes that share them.
-
-template <int NT, int STEP>
-struct IzGeom {
+// The separable synthesis marching through z.  A workgroup owns a 64 x 16
+// tile of (x, y) columns for `zs` consecutive output slices and walks its
+// input slices z0 - H .. z0 + zs - 1 + H once, in order.  Per input slice
+// and z filter k it forms the 2-D synthesis of the bands (*, *, k),
+//   S_k = sum_i corr_y[R_i]( sum_j corr_x[R_j] C_ijk ),
+// one y filter i at a time (the NF bands (i, *, k) staged in LDS, their x
+// pass summed over j in LDS, the y pass into registers), and adds
+// R_k[kz] * S_k to the 2H + 1 output slices that tap this input slice,
+// held in registers; the oldest is then complete and is stored.  Each
+// coefficient is read (zs + 2H) / zs times per halo position, and the
+// loads of the next band triple are in flight while the current one is
+// filtered.  Same in-place contract as k_iwt3 (`base` may alias u).
+template <int NT, int STEP, int TXW>
+struct ImGeom {
     static constexpr int CC = (NT - 1) / 2, H = CC * STEP;
-    static constexpr int PW = IW_TX + 2 * H, PH = IW_TY + 2 * H, P = PW * PH;
-    static constexpr size_t lds_for(int zc) { return ((size_t)zc * P + (size_t)zc * PH * IW_TX + P) * 4; }
-    // output slices per workgroup: 8 where their LDS fits 80 KB (the linear
-    // frame at dilation 1: 1.25 reads per coefficient, two workgroups per
-    // CU, 5% faster than 4 slices at three per CU), else 4
-    static constexpr int ZC = lds_for(8) <= 80 * 1024 ? 8 : 4, WZ = ZC + 2 * H;
-    static constexpr size_t lds = lds_for(ZC);
+    static constexpr int TX = TXW, ROWS = 4, TY = 4 * ROWS, NTH = 4 * TXW;   // four row groups of ROWS rows
+    static constexpr int PW = TX + 2 * H, PH = TY + 2 * H, P = PW * PH;
+    static constexpr int NPP = (P + NTH - 1) / NTH;   // halo positions per thread
+    template <int NF>
+    static constexpr size_t lds() { return 2 * ((size_t)NF * P + (size_t)PH * TX) * sizeof(float); }
 };
 
-template <int NF, int NT, int STEP, typename CT, typename XT>
-__global__ __launch_bounds__(256) void k_iwt3z(const CT* __restrict__ C, XT* u, int nx, int ny, int nz, FlTaps T,
-                                              float alpha, float beta, const float* base)
+template <int NF, int NT, int STEP, int TXW, typename CT, typename XT>
+__global__ __launch_bounds__(4 * TXW) void k_iwt3m(const CT* __restrict__ C, XT* u, int nx, int ny, int nz, int zs,
+                                                  FlTaps T, float alpha, float beta, const float* base,
+                                                  const CT* lp)
 {
-    using G = IzGeom<NT, STEP>;
-    constexpr int CC = G::CC, H = G::H, PW = G::PW, PH = G::PH, P = G::P, WZ = G::WZ, ZC = G::ZC;
-    constexpr int SX = PH * IW_TX;
-    __shared__ float tzs[ZC][P];    // z pass of the bands (i, j, *), per output slice
-    __shared__ float sxs[ZC][SX];   // x pass, summed over j, per output slice
-    __shared__ int poff[P];            // halo position -> wrapped offset in a slice
+    using G = ImGeom<NT, STEP, TXW>;
+    constexpr int CC = G::CC, H = G::H, PW = G::PW, PH = G::PH, P = G::P, NPP = G::NPP, TX = G::TX, TY = G::TY,
+                  ROWS = G::ROWS, NTH = G::NTH, NR = 2 * H + 1, NS = NF * NF;
+    // double-buffered: the bands (i, j, k), j = 0 .. NF-1, of one input
+    // slice on the tile and its halo, and their x pass summed over j
+    __shared__ float sb[2][NF][P];
+    __shared__ __align__(16) float sx[2][PH][TX];
     const size_t nxy = (size_t)nx * ny, nvox = nxy * nz;
-    const int tilesx = (nx + IW_TX - 1) / IW_TX;
-    // consecutive tiles (x fastest, then y, then z) on one XCD: the
-    // workgroups an XCD runs together are neighbours, so the cache lines
-    // their halos share (partial lines at the tile edges, the y and z halo
-    // rows) are fetched into that XCD's L2 once
-    const uint32_t lin = fl_xcd_remap(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
-    const uint32_t tile = lin % gridDim.x;
-    const int x0 = (int)(tile % tilesx) * IW_TX, y0 = (int)(tile / tilesx) * IW_TY;
-    const int z0 = (int)(lin / gridDim.x) * ZC;
-    const int tid = threadIdx.x;
-    for (int p = tid; p < P; p += 256) poff[p] = wrapi(y0 - H + p / PW, ny) * nx + wrapi(x0 - H + p % PW, nx);
-    size_t zoff[WZ];   // input slices z0 - H .. z0 + ZC - 1 + H (wrapped)
+    const int tilesx = (nx + TX - 1) / TX;
+    const uint32_t ntile = (uint32_t)tilesx * ((ny + TY - 1) / TY);
+    // consecutive tiles (x fastest, then y, then z) on one XCD
+    const uint32_t lin = fl_xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t tile = lin % ntile;
+    const int x0 = (int)(tile % tilesx) * TX, y0 = (int)(tile / tilesx) * TY;
+    const int z0 = (int)(lin / ntile) * zs, z1 = min(z0 + zs, nz);
+    const int tid = threadIdx.x, ox = tid % TX, oy = (tid / TX) * ROWS;
+    int off[NPP];   // this thread's halo positions -> wrapped offsets in a slice (-1: none)
 #pragma unroll
-    for (int w = 0; w < WZ; w++) zoff[w] = (size_t)wrapi(z0 - H + w, nz) * nxy;
-    __syncthreads();
-    float acc[ZC][IW_OUT];
+    for (int q = 0; q < NPP; q++) {
+        const int p = tid + NTH * q;
+        off[q] = p < P ? wrapi(y0 - H + p / PW, ny) * nx + wrapi(x0 - H + p % PW, nx) : -1;
+    }
+    // Steps s = 0 .. nsteps-1: input slice z0 - H + s / NS, z filter
+    // k = (s / NF) % NF, y filter i = s % NF.  A step stages its bands
+    // (phase A), forms their x pass (B) and adds their y pass to S_k (C);
+    // iteration t runs A of step t, B of step t - 1 and C of step t - 2
+    // between two barriers, on alternating buffers.  The raw coefficients
+    // of step t + 2 load into one of two register sets during iteration t.
+    const int nsteps = (z1 - z0 + 2 * H) * NS;
+    CT preA[NF][NPP], preB[NF][NPP];
+    auto fetch = [&](int s, CT (&pre)[NF][NPP]) {
+        const int sl = s / NS, k = (s / NF) % NF, i = s % NF;
+        const size_t zo = (size_t)wrapi(z0 - H + sl, nz) * nxy;
 #pragma unroll
-    for (int zz = 0; zz < ZC; zz++)
-#pragma unroll
-        for (int r = 0; r < IW_OUT; r++) acc[zz][r] = 0.0f;
-#pragma unroll 1
-    for (int i = 0; i < NF; i++) {
-#pragma unroll 1
         for (int j = 0; j < NF; j++) {
-            for (int p = tid; p < P; p += 256) {
-                const int off = poff[p];
-                float v[NF][WZ];
+            const CT* __restrict__ cz = band_ptr(C, i + NF * j + NF * NF * k, nvox, lp) + zo;
 #pragma unroll
-                for (int k = 0; k < NF; k++) {
-                    const CT* __restrict__ cb = C + (size_t)(i + NF * j + NF * NF * k) * nvox + off;
-#pragma unroll
-                    for (int w = 0; w < WZ; w++) v[k][w] = fl_load(cb, zoff[w]);
-                }
-#pragma unroll
-                for (int zz = 0; zz < ZC; zz++) {
-                    float t = 0.0f;
-#pragma unroll
-                    for (int k = 0; k < NF; k++)
-#pragma unroll
-                        for (int kz = 0; kz < NT; kz++) t = __builtin_fmaf(T.R[k][kz], v[k][zz + H + (kz - CC) * STEP], t);
-                    tzs[zz][p] = t;
-                }
-            }
-            __syncthreads();
-            for (int q = tid; q < ZC * SX; q += 256) {
-                const int zz = q / SX, q2 = q % SX;
-                const float* row = &tzs[zz][(q2 / IW_TX) * PW + q2 % IW_TX + H];
-                float s = 0.0f;
-#pragma unroll
-                for (int kx = 0; kx < NT; kx++) s = __builtin_fmaf(T.R[j][kx], row[(kx - CC) * STEP], s);
-                sxs[zz][q2] = j == 0 ? s : sxs[zz][q2] + s;
-            }
-            __syncthreads();
+            for (int q = 0; q < NPP; q++) pre[j][q] = off[q] >= 0 ? cz[off[q]] : CT{};
         }
+    };
+    float ring[NR][ROWS];   // ring[s]: output slice zin - H + s
+    float S[ROWS];
 #pragma unroll
-        for (int zz = 0; zz < ZC; zz++)
+    for (int q = 0; q < NR; q++)
 #pragma unroll
-            for (int r = 0; r < IW_OUT; r++) {
-                const int o = tid + 256 * r, ox = o % IW_TX, oy = o / IW_TX;
-                float s = 0.0f;
+        for (int r = 0; r < ROWS; r++) ring[q][r] = 0.0f;
+    // t: iteration; W: the buffer A of step t writes (t % 2)
+    auto iter = [&](int t, auto wbuf, CT (&pre)[NF][NPP]) {
+        constexpr int W = decltype(wbuf)::value, Rb = 1 - W;
+        if (t < nsteps) {   // A(t)
 #pragma unroll
-                for (int ky = 0; ky < NT; ky++)
-                    s = __builtin_fmaf(T.R[i][ky], sxs[zz][(oy + H + (ky - CC) * STEP) * IW_TX + ox], s);
-                acc[zz][r] += s;
-            }
-    }
+            for (int j = 0; j < NF; j++)
 #pragma unroll
-    for (int zz = 0; zz < ZC; zz++) {
-        const int z = z0 + zz;
-        if (z >= nz) break;
+                for (int q = 0; q < NPP; q++)
+                    if (tid + NTH * q < P) sb[W][j][tid + NTH * q] = fl_load(&pre[j][q], 0);
+            if (t + 2 < nsteps) fetch(t + 2, pre);
+        }
+        if (t >= 1 && t <= nsteps) {   // B(t - 1): four adjacent outputs per work item
+            for (int q = tid; q < PH * (TX / 4); q += NTH) {
+                const int y = q / (TX / 4), x = (q % (TX / 4)) * 4;
+                float a[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int r = 0; r < IW_OUT; r++) {
-            const int o = tid + 256 * r, x = x0 + o % IW_TX, y = y0 + o / IW_TX;
-            if (x < nx && y < ny) {
-                const size_t idx = (size_t)z * nxy + (size_t)y * nx + x;
-                float v = alpha * acc[zz][r];
-                if (base) v = __builtin_fmaf(beta, base[idx], v);
-                fl_store(u, idx, v);
+                for (int j = 0; j < NF; j++) {
+                    float v[4 + 2 * H];
+#pragma unroll
+                    for (int m = 0; m < 4 + 2 * H; m++) v[m] = sb[Rb][j][y * PW + x + m];
+#pragma unroll
+                    for (int o = 0; o < 4; o++)
+#pragma unroll
+                        for (int kx = 0; kx < NT; kx++) a[o] = __builtin_fmaf(T.R[j][kx], v[o + H + (kx - CC) * STEP], a[o]);
+                }
+                *reinterpret_cast<float4*>(&sx[Rb][y][x]) = make_float4(a[0], a[1], a[2], a[3]);
             }
         }
+        if (t >= 2) {   // C(t - 2)
+            const int s = t - 2, sl = s / NS, k = (s / NF) % NF, i = s % NF;
+            if (i == 0) {
+#pragma unroll
+                for (int r = 0; r < ROWS; r++) S[r] = 0.0f;
+            }
+#pragma unroll
+            for (int r = 0; r < ROWS; r++) {
+                float a = 0.0f;
+#pragma unroll
+                for (int ky = 0; ky < NT; ky++) a = __builtin_fmaf(T.R[i][ky], sx[W][oy + r + H + (ky - CC) * STEP][ox], a);
+                S[r] += a;
+            }
+            if (i == NF - 1) {
+                // S_k feeds output slice zin - (kz - CC)*STEP, ring slot H - (kz - CC)*STEP
+#pragma unroll
+                for (int kz = 0; kz < NT; kz++)
+#pragma unroll
+                    for (int r = 0; r < ROWS; r++)
+                        ring[H - (kz - CC) * STEP][r] =
+                            __builtin_fmaf(T.R[k][kz], S[r], ring[H - (kz - CC) * STEP][r]);
+                if (k == NF - 1) {
+                    // output slice zin - H has all its taps
+                    const int z = z0 - 2 * H + sl;
+                    if (z >= z0) {
+                        const int x = x0 + ox;
+#pragma unroll
+                        for (int r = 0; r < ROWS; r++) {
+                            const int y = y0 + oy + r;
+                            if (x < nx && y < ny) {
+                                const size_t idx = (size_t)z * nxy + (size_t)y * nx + x;
+                                float v = alpha * ring[0][r];
+                                if (base) v = __builtin_fmaf(beta, base[idx], v);
+                                fl_store(u, idx, v);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q + 1 < NR; q++)
+#pragma unroll
+                        for (int r = 0; r < ROWS; r++) ring[q][r] = ring[q + 1][r];
+#pragma unroll
+                    for (int r = 0; r < ROWS; r++) ring[NR - 1][r] = 0.0f;
+                }
+            }
+        }
+        __syncthreads();
+    };
+    fetch(0, preA);
+    if (nsteps > 1) fetch(1, preB);
+#pragma unroll 1
+    for (int t = 0; t < nsteps + 2; t += 2) {
+        iter(t, std::integral_constant<int, 0>{}, preA);
+        if (t + 1 < nsteps + 2) iter(t + 1, std::integral_constant<int, 1>{}, preB);
     }
+}
+
+// Launch of the marching synthesis: full-row tiles (TX = 256) where the
+// rows are 256 wide or a multiple of it and their LDS fits 160 KB (no x
+// halo: the taps wrap inside the tile), else 64-wide tiles.  Output slices
+// per workgroup: the longest run (fewer halo re-reads) that still gives at
+// least one workgroup per CU and `min_waves` waves, at least 4.
+template <int NF, int NT, int STEP, typename CT, typename XT>
+static void launch_iwt3m(const FlFrameDesc& f, const CT* C, XT* u, const FlTaps& T, float alpha, float beta,
+                         const float* base, const CT* lp, hipStream_t st)
+{
+    auto go = [&](auto txw) {
+        constexpr int TXW = decltype(txw)::value;
+        constexpr int TY = ImGeom<NT, STEP, TXW>::TY;
+        const unsigned tiles = (unsigned)(((f.nx + TXW - 1) / TXW) * ((f.ny + TY - 1) / TY));
+        const size_t min_groups = std::max<size_t>(256, (size_t)FL_IWT_MIN_WAVES / (TXW / 16));
+        int zs = 64;
+        while (zs > 4 && (size_t)tiles * ((f.nz + zs - 1) / zs) < min_groups) zs /= 2;
+        zs = std::min(zs, std::max(f.nz, 1));
+        const dim3 grid(tiles * (unsigned)((f.nz + zs - 1) / zs));
+        hipLaunchKernelGGL((k_iwt3m<NF, NT, STEP, TXW, CT, XT>), grid, dim3(4 * TXW), 0, st, C, u, f.nx, f.ny, f.nz,
+                           zs, T, alpha, beta, base, lp);
+    };
+    if constexpr (FL_IWT_FULL_ROWS && ImGeom<NT, STEP, 256>::template lds<NF>() <= 160 * 1024) {
+        if (f.nx % 256 == 0) {
+            go(std::integral_constant<int, 256>{});
+            return;
+        }
+    }
+    go(std::integral_constant<int, 64>{});
 }
 
 // ---------------------------------------------------------------- 2D ------
@@ -405,13 +497,13 @@ __global__ __launch_bounds__(256) void k_fwt2(const XT* __restrict__ u, CT* __re
 template <int NF, int NT, typename CT, typename XT>
 __global__ __launch_bounds__(256) void k_iwt2(const CT* __restrict__ C, XT* u, int nx, int ny,
                                              int step, FlTaps T, float alpha, float beta,
-                                             const float* base)
+                                             const float* base, const CT* lp)
 {
     const size_t npix = (size_t)nx * ny;
     const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (idx >= npix) return;
     const int x = (int)(idx % nx), y = (int)(idx / nx);
-    float v = alpha * fl_synth2<NF, NT>(C, npix, nx, ny, x, y, step, T);
+    float v = alpha * fl_synth2<NF, NT>(C, npix, nx, ny, x, y, step, T, lp);
     if (base) v = __builtin_fmaf(beta, base[idx], v);
     fl_store(u, idx, v);
 }
@@ -577,7 +669,6 @@ __global__ __launch_bounds__(256) void k_l1grad2(const float* __restrict__ u, CT
 
 // --------------------------------------------------------------- host side --
 
-#include "fl_framelet.h"
 
 // The filter taps of GenerateFrameletFilter.m:1-30, D[i] (decomposition)
 // and R[i] = flip(D[i]) (reconstruction), NT taps each, in double.
@@ -663,32 +754,24 @@ static hipError_t fwt_t(const FlFrameDesc& f, const void* u, void* C, int level,
 
 template <typename CT, typename XT>
 static hipError_t iwt_t(const FlFrameDesc& f, const void* C, void* u, int level, float alpha, float beta,
-                        const float* base, hipStream_t st)
+                        const float* base, const void* lpv, hipStream_t st)
 {
     const FlTaps T = fl_make_taps(f.kind);
     const int step = 1 << (level - 1);
     const size_t n = (size_t)f.nx * f.ny * (f.dim == 3 ? f.nz : 1);
+    const CT* lp = (const CT*)lpv;
     if (f.dim == 3)
         FL_FRAME_SWITCH(f.kind, {
-            // dilations 1 and 2: the z-window tiles (while their LDS fits
-            // 80 and 48 KB); then the separable tiles while the halo tile fits;
-            // the per-voxel gather beyond
-            const unsigned ztiles = (unsigned)(((f.nx + IW_TX - 1) / IW_TX) * ((f.ny + IW_TY - 1) / IW_TY));
-            if (step == 1 && IzGeom<NT, 1>::lds <= 80 * 1024) {
-                const int zc = (IzGeom<NT, 1>::ZC);
-                const dim3 zgrid(ztiles, (unsigned)((f.nz + zc - 1) / zc));
-                hipLaunchKernelGGL((k_iwt3z<NF, NT, 1, CT, XT>), zgrid, dim3(256), 0, st, (const CT*)C, (XT*)u, f.nx,
-                                   f.ny, f.nz, T, alpha, beta, base);
+            // dilations 1 and 2: the z-marching tiles; then the separable
+            // tiles while their halo tile fits 48 KB; the per-voxel gather
+            // beyond
+            if (step == 1) {
+                (launch_iwt3m<NF, NT, 1, CT, XT>)(f, (const CT*)C, (XT*)u, T, alpha, beta, base, lp, st);
                 return hipGetLastError();
             }
-            if constexpr (IzGeom<NT, 2>::lds <= 48 * 1024) {
-                if (step == 2) {
-                    const int zc = (IzGeom<NT, 2>::ZC);
-                    const dim3 zgrid(ztiles, (unsigned)((f.nz + zc - 1) / zc));
-                    hipLaunchKernelGGL((k_iwt3z<NF, NT, 2, CT, XT>), zgrid, dim3(256), 0, st, (const CT*)C, (XT*)u,
-                                       f.nx, f.ny, f.nz, T, alpha, beta, base);
-                    return hipGetLastError();
-                }
+            if (step == 2) {
+                (launch_iwt3m<NF, NT, 2, CT, XT>)(f, (const CT*)C, (XT*)u, T, alpha, beta, base, lp, st);
+                return hipGetLastError();
             }
             const int h = ((NT - 1) / 2) * step;
             const size_t lds = iw_lds_bytes<NF>(h);
@@ -699,16 +782,16 @@ static hipError_t iwt_t(const FlFrameDesc& f, const void* C, void* u, int level,
                 int zc = 1;
                 while (zc < 8 && (size_t)tiles * ((f.nz + 2 * zc - 1) / (2 * zc)) >= 2048) zc *= 2;
                 hipLaunchKernelGGL((k_iwt3s<NF, NT, CT, XT>), dim3(tiles, (f.nz + zc - 1) / zc), dim3(256), lds, st,
-                                   (const CT*)C, (XT*)u, f.nx, f.ny, f.nz, step, zc, T, alpha, beta, base);
+                                   (const CT*)C, (XT*)u, f.nx, f.ny, f.nz, step, zc, T, alpha, beta, base, lp);
             } else {
                 hipLaunchKernelGGL((k_iwt3<NF, NT, CT, XT>), dim3(nblk(n)), dim3(256), 0, st, (const CT*)C, (XT*)u,
-                                   f.nx, f.ny, f.nz, step, T, alpha, beta, base);
+                                   f.nx, f.ny, f.nz, step, T, alpha, beta, base, lp);
             }
         });
     else
         FL_FRAME_SWITCH(f.kind, hipLaunchKernelGGL((k_iwt2<NF, NT, CT, XT>), dim3(nblk(n)), dim3(256), 0, st,
                                                    (const CT*)C, (XT*)u, f.nx, f.ny, step, T, alpha, beta,
-                                                   base));
+                                                   base, lp));
     return hipGetLastError();
 }
 
@@ -723,13 +806,13 @@ hipError_t fl_launch_fwt(const FlFrameDesc& f, const void* u, int ut, void* C, i
 }
 
 hipError_t fl_launch_iwt(const FlFrameDesc& f, const void* C, int ct, void* u, int ut, int level, float alpha,
-                         float beta, const float* base, hipStream_t st)
+                         float beta, const float* base, hipStream_t st, const void* lp)
 {
-    if (ct == FL_F32 && ut == FL_F32) return iwt_t<float, float>(f, C, u, level, alpha, beta, base, st);
-    if (ct == FL_BF16 && ut == FL_F32) return iwt_t<bf16_t, float>(f, C, u, level, alpha, beta, base, st);
-    if (ct == FL_F16 && ut == FL_F32) return iwt_t<f16_t, float>(f, C, u, level, alpha, beta, base, st);
-    if (ct == FL_BF16 && ut == FL_BF16) return iwt_t<bf16_t, bf16_t>(f, C, u, level, alpha, beta, base, st);
-    if (ct == FL_F16 && ut == FL_F16) return iwt_t<f16_t, f16_t>(f, C, u, level, alpha, beta, base, st);
+    if (ct == FL_F32 && ut == FL_F32) return iwt_t<float, float>(f, C, u, level, alpha, beta, base, lp, st);
+    if (ct == FL_BF16 && ut == FL_F32) return iwt_t<bf16_t, float>(f, C, u, level, alpha, beta, base, lp, st);
+    if (ct == FL_F16 && ut == FL_F32) return iwt_t<f16_t, float>(f, C, u, level, alpha, beta, base, lp, st);
+    if (ct == FL_BF16 && ut == FL_BF16) return iwt_t<bf16_t, bf16_t>(f, C, u, level, alpha, beta, base, lp, st);
+    if (ct == FL_F16 && ut == FL_F16) return iwt_t<f16_t, f16_t>(f, C, u, level, alpha, beta, base, lp, st);
     return hipErrorInvalidValue;
 }
 

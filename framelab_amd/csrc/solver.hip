@@ -634,20 +634,19 @@ extern "C" int fl_framelet_adjoint(int dim, int kind, int level, int nx, int ny,
     }
     // FraRecMultiLevel2D.m:16-18 / 3D.m:14-16: C{L-1}{1} <- Rec(C{L}),
     // ..., u = Rec(C{1}).  The caller's coefficients are left untouched:
-    // each rebuilt level lives in one half of the work buffer (2*nb*n FP32),
-    // its low-pass band replaced by the reconstruction of the level above.
+    // each level's reconstruction goes to one of two volumes of the work
+    // buffer, and the synthesis of the level below reads it in place of
+    // its low-pass band.
     float* work = (float*)d_work;
     const float* C = (const float*)d_c;
-    const float* cur = C + (size_t)(level - 1) * nb * n;
+    const float* lp = nullptr;
     for (int L = level; L >= 2; L--) {
-        float* buf = work + (size_t)((level - L) & 1) * nb * n;
-        const float* lower = C + (size_t)(L - 2) * nb * n;
-        e = fl_launch_copy(lower + n, buf + n, (size_t)(nb - 1) * n, st);
-        if (e == hipSuccess) e = fl_launch_iwt(f, cur, FL_F32, buf, FL_F32, L, 1.0f, 0.0f, nullptr, st);
+        float* rec = work + (size_t)((level - L) & 1) * n;
+        e = fl_launch_iwt(f, C + (size_t)(L - 1) * nb * n, FL_F32, rec, FL_F32, L, 1.0f, 0.0f, nullptr, st, lp);
         if (e != hipSuccess) return fl_hip_fail(e, "framelet adjoint");
-        cur = buf;
+        lp = rec;
     }
-    e = fl_launch_iwt(f, cur, FL_F32, d_u, ut, 1, 1.0f, 0.0f, nullptr, st);
+    e = fl_launch_iwt(f, C, FL_F32, d_u, ut, 1, 1.0f, 0.0f, nullptr, st, lp);
     if (e != hipSuccess) return fl_hip_fail(e, "framelet adjoint");
     return fl_ok();
 }

@@ -77,7 +77,7 @@ class FrameletSystem:
         u = torch.empty(shape, dtype=out_dtype, device=c.device)
         work = None
         if self.level > 1:
-            work = torch.empty((2, self.nbands) + shape, dtype=torch.float32, device=c.device)
+            work = torch.empty((2,) + shape, dtype=torch.float32, device=c.device)
         _lib.check(_lib.lib().fl_framelet_adjoint(
             self.dim, self.kind, self.level, nx, ny, nz, C.c_void_p(c.data_ptr()), _DT[c.dtype],
             C.c_void_p(u.data_ptr()), _DT[u.dtype], C.c_void_p(work.data_ptr() if work is not None else 0),

@@ -176,7 +176,7 @@ int fl_framelet_forward(int dim, int kind, int level, int nx, int ny, int nz, co
 
 /* Synthesis W^T (FrameletSystem.adjointTransform, FrameletSystem.m:51-64 ->
  * FraRecMultiLevel{2D,3D}.m).  For level > 1 the coefficients must be FP32
- * and d_work must hold 2*NB*nz*ny*nx floats; d_c is not modified. */
+ * and d_work must hold 2*nz*ny*nx floats (two volumes); d_c is not modified. */
 int fl_framelet_adjoint(int dim, int kind, int level, int nx, int ny, int nz, const void *d_c,
                         int c_dtype, void *d_u, int u_dtype, void *d_work, void *stream);
 

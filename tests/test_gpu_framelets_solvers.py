@@ -31,9 +31,11 @@ def test_framelet_vs_oracle(gpu, oracle, kind, dim):
     shape = (16, 20, 24) if dim == 3 else (32, 40)
     u = rng.standard_normal(shape).astype(np.float32)
     D, R = oracle.framelet_filters(kind)
-    # 3D level 3: tiled synthesis with a 4-voxel halo (haar, linear), the
-    # per-voxel gather once the halo outgrows the tiles (cubic)
-    for level in ((1, 2, 3) if dim == 3 else (1, 2)):
+    # 3D levels 3-4: tiled synthesis with a 4- or 8-voxel halo (haar,
+    # linear), the per-voxel gather once the halo outgrows the tiles (cubic);
+    # every level below the top reads the reconstruction of the level above
+    # in place of its low-pass band
+    for level in ((1, 2, 3, 4) if dim == 3 else (1, 2, 3)):
         W = FrameletSystem(dim, kind, level)
         c = W.forwardTransform(torch.from_numpy(u).to(gpu)).frameletArray.cpu().numpy()
         ref = (oracle.fradec3d if dim == 3 else oracle.fradec2d)(u.astype(np.float64), D, level)
@@ -76,6 +78,31 @@ def test_framelet_partial_z_blocks(gpu, oracle, nz):
         want = oracle.frarec3d([cr[L].astype(np.float64).reshape((3, 3, 3) + u.shape).transpose(2, 1, 0, 3, 4, 5)
                                 for L in range(level)], R)
         assert _rel(got, want) < 2e-6, (nz, level)
+
+
+@pytest.mark.parametrize("kind", ["haar", "linear", "cubic"])
+def test_framelet_full_row_synthesis(gpu, oracle, kind):
+    """3-D synthesis on rows 256 and 512 wide (the full-row tiles, whose x
+    taps wrap inside the tile; row counts and slice counts not multiples of
+    the tiles'), levels 1-2 (dilations 1 and 2): arbitrary coefficients
+    against the FP64 oracle within 2e-6, and W^T W = I."""
+    from framelab_amd.framelets import FrameletSystem
+    rng = np.random.default_rng(11)
+    D, R = oracle.framelet_filters(kind)
+    nf = len(D)
+    for shape in ((7, 19, 256), (5, 9, 512)):
+        _say(f"case: framelet {kind} 3D full rows {shape}")
+        u = rng.standard_normal(shape).astype(np.float32)
+        for level in (1, 2):
+            W = FrameletSystem(3, kind, level)
+            c = W.forwardTransform(torch.from_numpy(u).to(gpu))
+            rec = W.adjointTransform(c).cpu().numpy()
+            assert _rel(rec, u) < 2e-6, (kind, shape, level)
+            cr = rng.standard_normal(tuple(c.frameletArray.shape)).astype(np.float32)
+            got = W.adjointTransform(torch.from_numpy(cr).to(gpu)).cpu().numpy()
+            want = oracle.frarec3d([cr[L].astype(np.float64).reshape((nf,) * 3 + shape).transpose(2, 1, 0, 3, 4, 5)
+                                    for L in range(level)], R)
+            assert _rel(got, want) < 2e-6, (kind, shape, level)
 
 
 def test_framelet_dot_test(gpu):
