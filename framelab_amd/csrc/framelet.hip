@@ -306,8 +306,12 @@ struct ImGeom {
     static constexpr int TX = TXW, ROWS = 4, TY = 4 * ROWS, NTH = 4 * TXW;   // four row groups of ROWS rows
     static constexpr int PW = TX + 2 * H, PH = TY + 2 * H, P = PW * PH;
     static constexpr int NPP = (P + NTH - 1) / NTH;   // halo positions per thread
+    // LDS row pitch: a multiple of 4 floats and at least TX + 4, so the x
+    // pass reads each row span as whole 16-byte words, conflict-free
+    static constexpr int PWP = ((PW + 3) / 4) * 4 < TX + 4 ? TX + 4 : ((PW + 3) / 4) * 4;
+    static constexpr int NV = (4 + 2 * H + 3) / 4;   // 16-byte words per x-pass span
     template <int NF>
-    static constexpr size_t lds() { return 2 * ((size_t)NF * P + (size_t)PH * TX) * sizeof(float); }
+    static constexpr size_t lds() { return 2 * ((size_t)NF * PH * PWP + (size_t)PH * TX) * sizeof(float); }
 };
 
 template <int NF, int NT, int STEP, int TXW, typename CT, typename XT>
@@ -317,10 +321,12 @@ __global__ __launch_bounds__(4 * TXW) void k_iwt3m(const CT* __restrict__ C, XT*
 {
     using G = ImGeom<NT, STEP, TXW>;
     constexpr int CC = G::CC, H = G::H, PW = G::PW, PH = G::PH, P = G::P, NPP = G::NPP, TX = G::TX, TY = G::TY,
-                  ROWS = G::ROWS, NTH = G::NTH, NR = 2 * H + 1, NS = NF * NF;
+                  ROWS = G::ROWS, NTH = G::NTH, NR = 2 * H + 1, NS = NF * NF, PWP = G::PWP, NV = G::NV;
+    static_assert(PWP % 4 == 0 && PWP >= 4 * NV + TX - 4, "x-pass spans must stay in their LDS row");
     // double-buffered: the bands (i, j, k), j = 0 .. NF-1, of one input
-    // slice on the tile and its halo, and their x pass summed over j
-    __shared__ float sb[2][NF][P];
+    // slice on the tile and its halo (rows of pitch PWP), and their x pass
+    // summed over j
+    __shared__ __align__(16) float sb[2][NF][PH * PWP];
     __shared__ __align__(16) float sx[2][PH][TX];
     const size_t nxy = (size_t)nx * ny, nvox = nxy * nz;
     const int tilesx = (nx + TX - 1) / TX;
@@ -331,11 +337,14 @@ __global__ __launch_bounds__(4 * TXW) void k_iwt3m(const CT* __restrict__ C, XT*
     const int x0 = (int)(tile % tilesx) * TX, y0 = (int)(tile / tilesx) * TY;
     const int z0 = (int)(lin / ntile) * zs, z1 = min(z0 + zs, nz);
     const int tid = threadIdx.x, ox = tid % TX, oy = (tid / TX) * ROWS;
-    int off[NPP];   // this thread's halo positions -> wrapped offsets in a slice (-1: none)
+    // this thread's halo positions -> wrapped offsets in a slice (-1: none)
+    // and places in an LDS band
+    int off[NPP], lix[NPP];
 #pragma unroll
     for (int q = 0; q < NPP; q++) {
         const int p = tid + NTH * q;
         off[q] = p < P ? wrapi(y0 - H + p / PW, ny) * nx + wrapi(x0 - H + p % PW, nx) : -1;
+        lix[q] = (p / PW) * PWP + p % PW;
     }
     // Steps s = 0 .. nsteps-1: input slice z0 - H + s / NS, z filter
     // k = (s / NF) % NF, y filter i = s % NF.  A step stages its bands
@@ -369,7 +378,7 @@ __global__ __launch_bounds__(4 * TXW) void k_iwt3m(const CT* __restrict__ C, XT*
             for (int j = 0; j < NF; j++)
 #pragma unroll
                 for (int q = 0; q < NPP; q++)
-                    if (tid + NTH * q < P) sb[W][j][tid + NTH * q] = fl_load(&pre[j][q], 0);
+                    if (tid + NTH * q < P) sb[W][j][lix[q]] = fl_load(&pre[j][q], 0);
             if (t + 2 < nsteps) fetch(t + 2, pre);
         }
         if (t >= 1 && t <= nsteps) {   // B(t - 1): four adjacent outputs per work item
@@ -378,9 +387,12 @@ __global__ __launch_bounds__(4 * TXW) void k_iwt3m(const CT* __restrict__ C, XT*
                 float a[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
                 for (int j = 0; j < NF; j++) {
-                    float v[4 + 2 * H];
+                    float v[4 * NV];
 #pragma unroll
-                    for (int m = 0; m < 4 + 2 * H; m++) v[m] = sb[Rb][j][y * PW + x + m];
+                    for (int m = 0; m < NV; m++) {
+                        const float4 w = *reinterpret_cast<const float4*>(&sb[Rb][j][y * PWP + x + 4 * m]);
+                        v[4 * m] = w.x; v[4 * m + 1] = w.y; v[4 * m + 2] = w.z; v[4 * m + 3] = w.w;
+                    }
 #pragma unroll
                     for (int o = 0; o < 4; o++)
 #pragma unroll
