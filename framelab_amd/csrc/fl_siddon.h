@@ -115,14 +115,14 @@ FL_HD bool sid_far(const SidAxis& a, int t)
     return a.on && (a.st > 0 ? t >= a.n : t < 0);
 }
 
-// Walk state of one axis: parameter of the next plane (INF once past the
-// last), that plane's coordinate index as a float (exact integer), and the
-// constants of the branch-free advance.
+// Walk state of one axis: parameter of the next plane (INF for an axis
+// without planes or already past its last one at the start), that plane's
+// coordinate index as a float (exact integer), and the constants of the
+// branch-free advance.
 struct SidNext {
     float a;     // alpha of the next plane
     float p;     // its index i (float)
     float st;    // +1 / -1
-    float past;  // index one beyond the last plane
     float r;     // 1/d, correctly rounded
 };
 
@@ -130,20 +130,24 @@ FL_HD void sid_next_init(SidNext& n, const SidAxis& ax, int j)
 {
     n.st = (float)ax.st;
     n.p = (float)(ax.i0 + ax.st * j);
-    n.past = (float)(ax.i0 + ax.st * (ax.n + 1));
     n.r = ax.on ? 1.0f / ax.d : 0.0f;
     n.a = (ax.on && j <= ax.n) ? sid_alpha(ax, j) : FL_INF;
 }
 
 // If this axis's next plane is the one just reached (alpha == an), move to
 // the following plane.  Branch-free; exact ties advance every tied axis.
+// Past an axis's last plane the advance goes on to the plane one cell
+// beyond it instead of ending that axis: the reference's list has no such
+// plane, but every segment after the last plane lies outside the volume on
+// that axis, and the walk stops at the first of them (its midpoint is at
+// least half a cell beyond the boundary once it reaches the extra plane),
+// so the visited segments are the same.
 template <bool S1>
 FL_HD void sid_next_step(SidNext& n, const SidAxis& ax, float an)
 {
     const float pn = n.p + n.st;
     const float num = (S1 ? pn : pn * ax.s) - ax.o;
-    float v = sid_div(num, ax.d, n.r);
-    v = (pn == n.past) ? FL_INF : v;
+    const float v = sid_div(num, ax.d, n.r);
     const bool adv = n.a == an;
     n.a = adv ? v : n.a;
     n.p = adv ? pn : n.p;
