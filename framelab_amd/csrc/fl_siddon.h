@@ -158,7 +158,7 @@ FL_HD void sid_next_step(SidNext& n, const SidAxis& ax, float an)
 // receives (linear voxel index (z*ny+y)*nx+x, alpha difference of the
 // segment) for every in-volume segment in traversal order; the length is
 // that difference times |d|.
-template <bool NZ3, bool DZ1, bool XYZ, class Visit>
+template <bool NZ3, bool DZ1, bool XYZ, bool ALL = false, class Visit>
 FL_HD void sid_walk_t(const SidAxis& ax, const SidAxis& ay, const SidAxis& az, float L, Visit&& visit)
 {
     constexpr bool dz_one = DZ1;
@@ -214,7 +214,21 @@ FL_HD void sid_walk_t(const SidAxis& ax, const SidAxis& ay, const SidAxis& az, f
             tz = (int)floorf(t + (float)az.nh);
         }
         bool in = sid_in(ax, tx) && sid_in(ay, ty) && (!NZ3 || sid_in(az, tz));
-        if (in) {
+        if constexpr (ALL) {
+            // every step, with its in-volume flag: visit(...) issues the
+            // step's work and returns a token that visit.commit consumes
+            // after this step's advance, so what the visit waits for (a
+            // gather) has the advance's arithmetic to complete behind
+            const auto tok = visit(tx, ty, tz, an - cur, in);
+            const bool stop = !in && (sid_far(ax, tx) || sid_far(ay, ty) || (NZ3 && sid_far(az, tz)));
+            sid_next_step<true>(qx, ax, an);
+            sid_next_step<true>(qy, ay, an);
+            if (NZ3) sid_next_step<DZ1>(qz, az, an);
+            cur = an;
+            visit.commit(tok);
+            if (stop) break;
+            continue;
+        } else if (in) {
             if constexpr (XYZ) visit(tx, ty, tz, an - cur);
             else visit((tz * ny + ty) * nx + tx, an - cur);
         } else {
@@ -233,6 +247,18 @@ FL_HD void sid_walk(const SidAxis& ax, const SidAxis& ay, const SidAxis& az, flo
 {
     if (dz_one) sid_walk_t<NZ3, true, false>(ax, ay, az, L, visit);
     else sid_walk_t<NZ3, false, false>(ax, ay, az, L, visit);
+}
+
+// Every step of the walk: tok = visit(tx, ty, tz, dal, in) for the
+// segments before and inside the volume (in = the cell is inside), up to
+// the first one beyond it, and visit.commit(tok) once the step's plane
+// advance is issued.
+template <class Visit>
+FL_HD void sid_walk_all3(const SidAxis& ax, const SidAxis& ay, const SidAxis& az, float L, bool dz_one,
+                         Visit&& visit)
+{
+    if (dz_one) sid_walk_t<true, true, true, true>(ax, ay, az, L, visit);
+    else sid_walk_t<true, false, true, true>(ax, ay, az, L, visit);
 }
 
 // As sid_walk, but the visitor receives the cell coordinates (tx, ty, tz)

@@ -139,11 +139,29 @@ __global__ __launch_bounds__(TA* TB) void k_fwd_cone(const VT* __restrict__ X, Y
     sid_axis_init(ay, r.y1, r.dy, 1.0f, g.ny);
     sid_axis_init(az, r.z1, r.dz, g.dz0, g.nz);
     const float L = r.L;
-    float acc = 0.0f;
-    sid_walk_xyz<true>(ax, ay, az, L, g.dz0 == 1.0f, [&](int tx, int ty, int tz, float dal) {
-        float Si = dal * L;
-        acc = acc + Si * fl_load(x, (size_t)br.at(tx, ty, tz));
-    });
+    // y = y + S*x in walk order.  Every step issues its gather (outside the
+    // volume from cell 0, whose term is then +0) before the step's plane
+    // advance and adds its term after it, so the gather has the advance's
+    // arithmetic to land behind.  Adding +0 leaves a sum that starts at +0
+    // unchanged: the result is bit-identical to adding the in-volume terms.
+    struct Term {
+        float S;
+        VT x;
+        bool in;
+    };
+    struct Visit {
+        const VT* __restrict__ x;
+        const Bricks<BX, BY>& br;
+        float L;
+        float acc;
+        __device__ Term operator()(int tx, int ty, int tz, float dal, bool in) const
+        {
+            return Term{dal * L, x[in ? br.at(tx, ty, tz) : 0u], in};
+        }
+        __device__ void commit(const Term& t) { acc = acc + (t.in ? t.S * fl_load(&t.x, 0) : 0.0f); }
+    } visit{x, br, L, 0.0f};
+    sid_walk_all3(ax, ay, az, L, g.dz0 == 1.0f, visit);
+    const float acc = visit.acc;
     fl_store(Y, ((size_t)slot * g.nb + ib) * g.na + ia, acc * g.scale);
 }
 
