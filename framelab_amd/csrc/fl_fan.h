@@ -24,12 +24,27 @@ __device__ __forceinline__ float fl_fan_ray_sum(const VT* __restrict__ x, const 
     sid_axis_init(ax, x1, dx, 1.0f, g.nx);
     sid_axis_init(ay, y1, dy, 1.0f, g.ny);
     sid_axis_init(az, 0.0f, 0.0f, 1.0f, 2);
-    float acc = 0.0f;
-    sid_walk<false>(ax, ay, az, L, true, [&](int J, float dal) {
-        float Si = dal * L;
-        acc = acc + Si * fl_load(x, (size_t)J);
-    });
-    return acc * g.scale;
+    // y = y + S*x in walk order, each step's gather issued before the
+    // step's plane advance and its term added after it (outside the image
+    // from pixel 0, with the term +0, which leaves the sum bit-identical)
+    struct Term {
+        float S;
+        VT x;
+        bool in;
+    };
+    struct Visit {
+        const VT* __restrict__ x;
+        int nx;
+        float L;
+        float acc;
+        __device__ Term operator()(int tx, int ty, int, float dal, bool in) const
+        {
+            return Term{dal * L, x[in ? ty * nx + tx : 0], in};
+        }
+        __device__ void commit(const Term& t) { acc = acc + (t.in ? t.S * fl_load(&t.x, 0) : 0.0f); }
+    } visit{x, g.nx, L, 0.0f};
+    sid_walk_all2(ax, ay, az, L, visit);
+    return visit.acc * g.scale;
 }
 
 // Gao column walk of fan ray (iv, id), scaled (Ax_fan_mf_cpu_new.cpp:28-119).

@@ -261,6 +261,13 @@ FL_HD void sid_walk_all3(const SidAxis& ax, const SidAxis& ay, const SidAxis& az
     else sid_walk_t<true, false, true, true>(ax, ay, az, L, visit);
 }
 
+// The same for a 2-D walk (tz = 0).
+template <class Visit>
+FL_HD void sid_walk_all2(const SidAxis& ax, const SidAxis& ay, const SidAxis& az, float L, Visit&& visit)
+{
+    sid_walk_t<false, true, true, true>(ax, ay, az, L, visit);
+}
+
 // As sid_walk, but the visitor receives the cell coordinates (tx, ty, tz)
 // instead of the linear index.
 template <bool NZ3, class Visit>
