@@ -19,6 +19,17 @@ def bits_equal(a, b):
     return a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+def dot_gap(y, Y, X, Xt):
+    """<y, Y> - <X, Xt> for y = A X and Xt = A^T Y, over the scale of the
+    terms, max(sum |y||Y|, sum |X||Xt|) (FP64): the rounding of the FP32
+    operators is relative to that scale, not to the dot product itself,
+    which random signs can cancel to a small fraction of it."""
+    y, Y, X, Xt = (np.asarray(v, np.float64).ravel() for v in (y, Y, X, Xt))
+    lhs, rhs = float(np.dot(y, Y)), float(np.dot(X, Xt))
+    scale = max(float(np.dot(np.abs(y), np.abs(Y))), float(np.dot(np.abs(X), np.abs(Xt))), 1e-300)
+    return abs(lhs - rhs) / scale
+
+
 def first_mismatch(a, b):
     a = np.asarray(a, np.float32).ravel()
     b = np.asarray(b, np.float32).ravel()

@@ -3,7 +3,7 @@ near and far sources, coarse and fine detector pitches, detector offsets,
 non-unit voxel height, helical orbits, several image frames, fan and cone.
 On each, the GPU forward and exact adjoint equal the oracle bit for bit
 (the reference's float semantics), and the forward and matched adjoint
-pass the dot test.  The fixed-geometry tests pin the named configurations;
+pass the dot test (gap over the scale of its terms).  The fixed-geometry tests pin the named configurations;
 this sweep looks for the edge cases between them."""
 import math
 import sys
@@ -12,11 +12,11 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import bits_equal, first_mismatch
+from helpers import bits_equal, dot_gap, first_mismatch
 
 pytestmark = pytest.mark.gpu
 
-N_CONE, N_FAN = 40, 12
+N_CONE, N_FAN = 300, 100
 
 
 def _say(msg):
@@ -92,9 +92,7 @@ def test_random_cone(gpu, oracle, seed):
     xo = oracle.atx_cone_f32(Y, p)
     assert bits_equal(x, xo), first_mismatch(x, xo)
     xm = P.adjoint(Yt, kind=ADJ_MATCHED).cpu().numpy().ravel()
-    lhs = float(np.dot(y.astype(np.float64), Y.astype(np.float64)))
-    rhs = float(np.dot(X.astype(np.float64), xm.astype(np.float64)))
-    assert abs(lhs - rhs) <= 1e-5 * abs(lhs), (lhs, rhs)
+    assert dot_gap(y, Y, X, xm) <= 2e-6
 
 
 @pytest.mark.parametrize("seed", range(N_FAN))
@@ -114,9 +112,7 @@ def test_random_fan(gpu, oracle, seed):
     xo = oracle.atx_fan_f32(Y, p)
     assert bits_equal(x, xo), first_mismatch(x, xo)
     xm = P.adjoint(Yt, kind=ADJ_MATCHED).cpu().numpy().ravel()
-    lhs = float(np.dot(y.astype(np.float64), Y.astype(np.float64)))
-    rhs = float(np.dot(X.astype(np.float64), xm.astype(np.float64)))
-    assert abs(lhs - rhs) <= 1e-5 * abs(lhs), (lhs, rhs)
+    assert dot_gap(y, Y, X, xm) <= 2e-6
 
 
 @pytest.mark.parametrize("seed", range(N_FAN))
@@ -138,7 +134,7 @@ def test_random_fan_finite_beam(gpu, oracle, seed):
     assert bits_equal(x, xo), first_mismatch(x, xo)
 
 
-@pytest.mark.parametrize("seed", range(24))
+@pytest.mark.parametrize("seed", range(96))
 def test_random_framelets(gpu, oracle, seed):
     """Framelet analysis and synthesis on random sizes, odd and tiny ones
     included (the circular taps of the higher levels then wrap more than
