@@ -674,7 +674,7 @@ def main():
     # per-operator algorithmic bytes (per-view streaming model): nv*(Nvox*sx + Ndet*sy)
     B_op = nv * (nvox * 4 + ndet * 4)
     dom_ms, dom_name = ((mAt, "k_adj_cone_zc<float,float,4>") if mAt >= mA
-                        else (mA, "k_fwd_cone<float,float,16,16,8,8,8>"))
+                        else (mA, "k_fwd_cone<float,float,16,16,8,4,4,4>"))
     achieved = B_op / (dom_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom_name), "kernel": dom_name,

@@ -307,9 +307,10 @@ extern "C" int fl_plan_create(fl_plan* out, const fl_geometry* g)
     p->h_zdet = zdet;
     p->h_sdz = sdz;
     if (cone) {
-        // the forward gathers from a bricked copy of its input (8x8x1 bricks,
-        // padded to whole bricks; 4-byte elements cover every storage type)
-        const size_t fr = (size_t)((g->nx + 7) / 8) * ((g->ny + 7) / 8) * 64 * (size_t)g->nz;
+        // the forward gathers from a bricked copy of its input (padded to
+        // whole bricks of the largest layout; 4-byte elements cover every
+        // storage type)
+        const size_t fr = fl_bricked_frame_elems(g->nx, g->ny, g->nz);
         if (hipMalloc(&p->d_bricks, fr * (size_t)g->nt * 4) != hipSuccess) rc = fl_fail(FL_ERR_ALLOC, "brick buffer");
         else if (hipEventCreateWithFlags(&p->fwd_done, hipEventDisableTiming) != hipSuccess)
             rc = fl_fail(FL_ERR_HIP, "forward ordering event");
@@ -327,7 +328,9 @@ extern "C" int fl_plan_create(fl_plan* out, const fl_geometry* g)
     f.nx = g->nx; f.ny = g->ny; f.nz = cone ? g->nz : 0; f.na = g->na; f.nb = g->nb;
     f.nslots = g->nv; f.nvox = (size_t)nvox; f.nframes = g->nt;
     f.bricks = p->d_bricks;
-    f.tile = cone ? 1 : 0;
+    // 16 x 16-ray workgroups of 8 x 8-ray waves gathering from 4 x 4 x 4
+    // bricks: the fastest shape measured at 256^3 (fl_plan_autotune retimes)
+    f.tile = cone ? 6 : 0;
 
     AdjGeom& a = p->ag;
     a.cphi = p->d_cphi; a.sphi = p->d_sphi; a.cdet = p->d_cdet; a.sdet = p->d_sdet;
@@ -480,7 +483,7 @@ extern "C" int fl_plan_autotune(fl_plan p, int* fwd_tile, float* fwd_ms)
     return fl_ok();
 }
 
-// Selects a cone-forward ray-tile shape by index (0..5).
+// Selects a cone-forward ray-tile shape by index (0..FL_FWD_NTILES-1).
 extern "C" int fl_plan_set_tile(fl_plan p, int tile)
 {
     if (!p || tile < 0 || tile >= FL_FWD_NTILES) return fl_fail(FL_ERR_INVALID, "bad plan or tile %d", tile);

@@ -22,7 +22,7 @@ struct fl_plan_s {
     int *d_slot_view = nullptr, *d_view_frame = nullptr, *d_idY = nullptr, *d_Nv = nullptr;
     int* d_idS = nullptr;         // adjoint: sinogram slot per frame-list entry (AdjGeom::id_S)
     bool slots_ok = true;         // every frame-list view has a forward output slot to read
-    void* d_bricks = nullptr;   // cone: bricked copy of the forward input (nt frames, 8x8 bricks, 4-byte)
+    void* d_bricks = nullptr;   // cone: bricked copy of the forward input (nt frames, bricks of the launch shape, 4-byte)
     // forwards share d_bricks: each one waits for the previous one (on
     // whatever stream it ran) through this event, recorded after it
     hipEvent_t fwd_done = nullptr;
@@ -48,7 +48,9 @@ struct fl_plan_s {
     size_t ws_bytes = 0;
 };
 
-constexpr int FL_FWD_NTILES = 6;   // cone forward ray-tile shapes (siddon.hip)
+constexpr int FL_FWD_NTILES = 8;   // cone forward ray-tile shapes (siddon.hip)
+// elements of the largest bricked frame of any of them (siddon.hip)
+size_t fl_bricked_frame_elems(int nx, int ny, int nz);
 // detector coordinates may stray from a uniform grid by at most this many
 // cells for the voxel-driven adjoints (their windows widen by 0.05 cells)
 constexpr double FL_DET_DEV_MAX = 0.01;

@@ -63,18 +63,19 @@ __device__ __forceinline__ RayGeo fan_ray(const FwdGeom& g, int iv, int id)
 }  // namespace
 
 // Bricked volume layout for the ray gathers: the frame is cut into
-// BX x BY x 1 bricks stored contiguously (x fastest inside a brick, bricks
-// x-major), so a cache line holds a compact patch of voxels whatever the
-// view angle and neighbouring rays share lines.  BX = BY = 1 is the plain
-// layout.  Padding to whole bricks is never read.
-template <int BX, int BY>
+// BX x BY x BZ bricks stored contiguously (x fastest inside a brick, then
+// y, then z; bricks x-major, then y, then z), so a cache line holds a
+// compact patch of voxels whatever the view angle and neighbouring rays
+// share lines.  BX = BY = BZ = 1 is the plain layout.  Padding to whole
+// bricks is never read.
+template <int BX, int BY, int BZ = 1>
 struct Bricks {
     int nbx, nby;
-    unsigned row, plane;   // elements per row of bricks / per z plane
+    unsigned row, plane;   // elements per row of bricks / per slab of BZ z planes
     size_t frame;          // elements per bricked frame
     __host__ __device__ Bricks(int nx, int ny, int nz)
-        : nbx((nx + BX - 1) / BX), nby((ny + BY - 1) / BY), row((unsigned)nbx * BX * BY),
-          plane(row * (unsigned)nby), frame((size_t)plane * nz) {}
+        : nbx((nx + BX - 1) / BX), nby((ny + BY - 1) / BY), row((unsigned)nbx * BX * BY * BZ),
+          plane(row * (unsigned)nby), frame((size_t)plane * ((nz + BZ - 1) / BZ)) {}
     // (tx, ty, tz) in the volume, so non-negative: unsigned shifts and masks,
     // and full-rate 24-bit multiplies -- bricked layouts are launched only
     // where every factor fits 24 bits (bricked_fits_u24), so no branch here
@@ -83,25 +84,33 @@ struct Bricks {
     __device__ __forceinline__ unsigned at(int tx, int ty, int tz) const
     {
         const unsigned x = (unsigned)tx, y = (unsigned)ty, z = (unsigned)tz;
-        if (BX == 1 && BY == 1) return (z * nby + y) * nbx + x;
-        const unsigned in = (x / BX) * (BX * BY) + (y % BY) * BX + x % BX;
-        return __umul24(z, plane) + __umul24(y / BY, row) + in;
+        if (BX == 1 && BY == 1 && BZ == 1) return (z * nby + y) * nbx + x;
+        const unsigned in = (x / BX) * (BX * BY * BZ) + ((z % BZ) * BY + y % BY) * BX + x % BX;
+        return __umul24(z / BZ, plane) + __umul24(y / BY, row) + in;
     }
 };
 
-// The bricked layouts' index arithmetic needs a brick plane (8 x 8 bricks,
-// the largest) of under 2^24 elements and under 2^24 planes.
+// Elements of the largest bricked frame of any launch shape (plan buffers).
+size_t fl_bricked_frame_elems(int nx, int ny, int nz)
+{
+    return std::max(Bricks<8, 8>(nx, ny, nz).frame,
+                    std::max(Bricks<4, 4, 4>(nx, ny, nz).frame, Bricks<4, 4, 2>(nx, ny, nz).frame));
+}
+
+// The bricked layouts' index arithmetic needs a brick slab of under 2^24
+// elements and under 2^24 slabs.
+template <int BX, int BY, int BZ>
 static bool bricked_fits_u24(int nx, int ny, int nz)
 {
-    const Bricks<8, 8> br(nx, ny, nz);
+    const Bricks<BX, BY, BZ> br(nx, ny, nz);
     return br.plane < (1u << 24) && nz < (1 << 24);
 }
 
-template <typename VT, int BX, int BY>
+template <typename VT, int BX, int BY, int BZ>
 __global__ __launch_bounds__(256) void k_to_bricks(const VT* __restrict__ x, VT* __restrict__ xb, int nx, int ny,
                                                   int nz, size_t nvox, size_t n)
 {
-    const Bricks<BX, BY> br(nx, ny, nz);
+    const Bricks<BX, BY, BZ> br(nx, ny, nz);
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
         const size_t f = i / nvox, v = i % nvox;
         const int tx = (int)(v % nx), ty = (int)((v / nx) % ny), tz = (int)(v / ((size_t)nx * ny));
@@ -112,8 +121,8 @@ __global__ __launch_bounds__(256) void k_to_bricks(const VT* __restrict__ x, VT*
 // A workgroup covers a TA x TB patch of detector cells of one view; each
 // wave covers a WA x (64/WA) sub-patch, so that its 64 rays run close
 // together through the volume and their gathers share cache lines.  X is
-// the input in the BX x BY brick layout.
-template <typename VT, typename YT, int TA, int TB, int WA, int BX, int BY>
+// the input in the BX x BY x BZ brick layout.
+template <typename VT, typename YT, int TA, int TB, int WA, int BX, int BY, int BZ>
 __global__ __launch_bounds__(TA* TB) void k_fwd_cone(const VT* __restrict__ X, YT* __restrict__ Y,
                                                     FwdGeom g)
 {
@@ -130,7 +139,7 @@ __global__ __launch_bounds__(TA* TB) void k_fwd_cone(const VT* __restrict__ X, Y
     const int ib = (int)(t / tiles_a) * TB + (w / NWA) * WB + (l / WA);
     if (slot >= g.nslots || ia >= g.na || ib >= g.nb) return;
     const int iv = g.slot_view[slot];
-    const Bricks<BX, BY> br(g.nx, g.ny, g.nz);
+    const Bricks<BX, BY, BZ> br(g.nx, g.ny, g.nz);
     const VT* __restrict__ x = X + (size_t)g.view_frame[iv] * br.frame;
 
     RayGeo r = cone_ray(g, iv, ia, ib);
@@ -151,7 +160,7 @@ __global__ __launch_bounds__(TA* TB) void k_fwd_cone(const VT* __restrict__ X, Y
     };
     struct Visit {
         const VT* __restrict__ x;
-        const Bricks<BX, BY>& br;
+        const Bricks<BX, BY, BZ>& br;
         float L;
         float acc;
         __device__ Term operator()(int tx, int ty, int tz, float dal, bool in) const
@@ -209,43 +218,47 @@ __global__ void k_trace(FwdGeom g, int cone, int nrays, const int* __restrict__ 
 
 // ---- host launchers ------------------------------------------------------
 
-template <typename VT, typename YT, int TA, int TB, int WA, int BX, int BY>
+template <typename VT, typename YT, int TA, int TB, int WA, int BX, int BY, int BZ = 1>
 static hipError_t launch_cone_shape(const void* X, void* Y, const FwdGeom& g, hipStream_t st)
 {
     uint32_t tiles = ((g.na + TA - 1) / TA) * ((g.nb + TB - 1) / TB);
     uint32_t grid = tiles * (uint32_t)g.nslots;
     if (grid == 0) return hipSuccess;
     const VT* src = (const VT*)X;
-    if (BX > 1 || BY > 1) {
+    if (BX > 1 || BY > 1 || BZ > 1) {
         // nt frames into the plan's brick buffer (sized for 4-byte elements)
         const int nt = g.nframes;
         const size_t n = g.nvox * (size_t)nt;
         const unsigned gb = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
-        hipLaunchKernelGGL((k_to_bricks<VT, BX, BY>), dim3(gb), dim3(256), 0, st, (const VT*)X, (VT*)g.bricks,
+        hipLaunchKernelGGL((k_to_bricks<VT, BX, BY, BZ>), dim3(gb), dim3(256), 0, st, (const VT*)X, (VT*)g.bricks,
                            g.nx, g.ny, g.nz, g.nvox, n);
         src = (const VT*)g.bricks;
     }
-    hipLaunchKernelGGL((k_fwd_cone<VT, YT, TA, TB, WA, BX, BY>), dim3(grid), dim3(TA * TB), 0, st, src, (YT*)Y,
-                       g);
+    hipLaunchKernelGGL((k_fwd_cone<VT, YT, TA, TB, WA, BX, BY, BZ>), dim3(grid), dim3(TA * TB), 0, st, src,
+                       (YT*)Y, g);
     return hipGetLastError();
 }
 
 // Launch shapes, indexed by FwdGeom::tile (fl_plan_autotune times them):
-// workgroup TA x TB rays, waves WA x 64/WA, volume bricks BX x BY x 1.
+// workgroup TA x TB rays, waves WA x 64/WA, volume bricks BX x BY x BZ.
 template <typename VT, typename YT>
 static hipError_t launch_cone_t(const void* X, void* Y, const FwdGeom& g, hipStream_t st)
 {
     if (g.tile != 0 && g.tile != 3 && !g.bricks) return hipErrorInvalidValue;
-    // planes too large for the bricks' 24-bit index arithmetic take the
+    // slabs too large for the bricks' 24-bit index arithmetic take the
     // plain layout of the same workgroup shape family
-    const int tile = (g.tile == 0 || g.tile == 3 || bricked_fits_u24(g.nx, g.ny, g.nz)) ? g.tile : 3;
+    const bool fits = bricked_fits_u24<8, 8, 1>(g.nx, g.ny, g.nz) && bricked_fits_u24<4, 4, 4>(g.nx, g.ny, g.nz) &&
+                      bricked_fits_u24<4, 4, 2>(g.nx, g.ny, g.nz);
+    const int tile = (g.tile == 0 || g.tile == 3 || fits) ? g.tile : 3;
     switch (tile) {
     case 0: return launch_cone_shape<VT, YT, 64, 4, 64, 1, 1>(X, Y, g, st);
     case 1: return launch_cone_shape<VT, YT, 16, 16, 8, 8, 8>(X, Y, g, st);
     case 2: return launch_cone_shape<VT, YT, 32, 8, 8, 8, 8>(X, Y, g, st);
     case 3: return launch_cone_shape<VT, YT, 16, 16, 8, 1, 1>(X, Y, g, st);
     case 4: return launch_cone_shape<VT, YT, 16, 16, 8, 4, 4>(X, Y, g, st);
-    default: return launch_cone_shape<VT, YT, 64, 4, 64, 8, 8>(X, Y, g, st);
+    case 5: return launch_cone_shape<VT, YT, 64, 4, 64, 8, 8>(X, Y, g, st);
+    case 6: return launch_cone_shape<VT, YT, 16, 16, 8, 4, 4, 4>(X, Y, g, st);
+    default: return launch_cone_shape<VT, YT, 16, 16, 8, 4, 4, 2>(X, Y, g, st);
     }
 }
 

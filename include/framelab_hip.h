@@ -133,7 +133,9 @@ int fl_plan_sizes(fl_plan plan, long long *nvox, int *nt, long long *ndet, int *
 
 /* Times the cone-forward ray-tile shapes on this plan's geometry and keeps
  * the fastest (outputs optional; fan plans: no-op).  fl_plan_set_tile picks
- * a shape by index 0..5.  No reference counterpart (launch tuning). */
+ * a shape by index 0..7; a cone plan starts at shape 6 (16 x 16-ray
+ * workgroups gathering from 4 x 4 x 4 voxel bricks).  No reference
+ * counterpart (launch tuning). */
 int fl_plan_autotune(fl_plan plan, int *fwd_tile, float *fwd_ms);
 int fl_plan_set_tile(fl_plan plan, int tile);
 
