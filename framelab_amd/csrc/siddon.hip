@@ -98,12 +98,14 @@ size_t fl_bricked_frame_elems(int nx, int ny, int nz)
 }
 
 // The bricked layouts' index arithmetic needs a brick slab of under 2^24
-// elements and under 2^24 slabs.
+// elements, under 2^24 slabs and, as its in-frame offsets are 32-bit, a
+// bricked frame of at most 2^32 elements (the padding to whole bricks can
+// make a thin volume's frame several times its voxel count).
 template <int BX, int BY, int BZ>
 static bool bricked_fits_u24(int nx, int ny, int nz)
 {
     const Bricks<BX, BY, BZ> br(nx, ny, nz);
-    return br.plane < (1u << 24) && nz < (1 << 24);
+    return br.plane < (1u << 24) && nz < (1 << 24) && br.frame <= ((size_t)1 << 32);
 }
 
 template <typename VT, int BX, int BY, int BZ>
