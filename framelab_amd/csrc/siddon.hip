@@ -81,12 +81,20 @@ struct Bricks {
     // where every factor fits 24 bits (bricked_fits_u24), so no branch here
     // (a branch on the plane size made the compiler fall back to the
     // quarter-rate 32-bit multiply on every step of the walk)
+    //
+    // The offset (x/BX)*BV + x%BX + (y/BY)*row + (y%BY)*BX + (z/BZ)*plane +
+    // (z%BZ)*BX*BY (BV = BX*BY*BZ) is formed as
+    //   x + BX*y + BX*BY*z + (x/BX)*(BV-BX) + (y/BY)*(row-BX*BY) + (z/BZ)*(plane-BV)
+    // (x%BX = x - (x/BX)*BX, likewise y and z): two shift-adds, three shifts
+    // and three 24-bit multiply-adds.  Every partial sum is taken modulo 2^32
+    // and the offset itself is below 2^32, so it comes out exact.
     __device__ __forceinline__ unsigned at(int tx, int ty, int tz) const
     {
         const unsigned x = (unsigned)tx, y = (unsigned)ty, z = (unsigned)tz;
         if (BX == 1 && BY == 1 && BZ == 1) return (z * nby + y) * nbx + x;
-        const unsigned in = (x / BX) * (BX * BY * BZ) + ((z % BZ) * BY + y % BY) * BX + x % BX;
-        return __umul24(z / BZ, plane) + __umul24(y / BY, row) + in;
+        constexpr unsigned BV = BX * BY * BZ;
+        const unsigned lin = x + BX * y + BX * BY * z;
+        return __umul24(z / BZ, plane - BV) + __umul24(y / BY, row - BX * BY) + __umul24(x / BX, BV - BX) + lin;
     }
 };
 
@@ -167,7 +175,9 @@ __global__ __launch_bounds__(TA* TB) void k_fwd_cone(const VT* __restrict__ X, Y
         float acc;
         __device__ Term operator()(int tx, int ty, int tz, float dal, bool in) const
         {
-            return Term{dal * L, x[in ? br.at(tx, ty, tz) : 0u], in};
+            // offset formed for every lane and masked to cell 0 outside
+            // the volume (no branch around the offset arithmetic)
+            return Term{dal * L, x[br.at(tx, ty, tz) & (0u - (unsigned)in)], in};
         }
         __device__ void commit(const Term& t) { acc = acc + (t.in ? t.S * fl_load(&t.x, 0) : 0.0f); }
     } visit{x, br, L, 0.0f};
