@@ -221,7 +221,9 @@ __global__ __launch_bounds__(256, FL_MATCHED_WG_PER_CU) void k_adj_matched_cone(
                         const int q = kb * nA + ka;
                         const float Si = matched_coef(inx, outx, iny, outy, inz, outz, ax, ay, az, ix, iy, iz,
                                                       dz_one, t_L[q]);
-                        acc[k] = acc[k] + Si * t_y[q];   // (+-0 for a miss: the sum is unchanged)
+                        // one fused multiply-add (no reference rounding to keep;
+                        // +-0 for a miss leaves the sum unchanged)
+                        acc[k] = __builtin_fmaf(Si, t_y[q], acc[k]);
                     }
                 }
             }
@@ -244,7 +246,7 @@ __global__ __launch_bounds__(256, FL_MATCHED_WG_PER_CU) void k_adj_matched_cone(
                         const float L = sqrtf(dx * dx + dy * dy + dz * dz);
                         const float Si = matched_coef(sx.in, sx.out, sy.in, sy.out, sv.in, sv.out, ax, ay, az, ix,
                                                       iy, iz, dz_one, L);
-                        if (Si != 0.0f) acc[k] = acc[k] + Si * fl_load(y, (size_t)ib * na + ia);
+                        if (Si != 0.0f) acc[k] = __builtin_fmaf(Si, fl_load(y, (size_t)ib * na + ia), acc[k]);
                     }
                 }
             }
