@@ -76,9 +76,9 @@ __global__ __launch_bounds__(256, FL_MATCHED_WG_PER_CU) void k_adj_matched_cone(
 {
     constexpr int BZ = 4 * KZ;
     __shared__ float t_ax[MF_A][9], t_ay[MF_A][9];
-    __shared__ float t_dx[MF_A], t_dy[MF_A], t_rx[MF_A], t_ry[MF_A];
+    __shared__ float t_dx[MF_A], t_dy[MF_A], t_rx[MF_A], t_ry[MF_A], t_dxy2[MF_A];
     __shared__ float2 t_zio[MF_B][BZ];   // per row and brick slice: z entry / exit plane parameters
-    __shared__ float t_dz[MF_B], t_rz[MF_B];
+    __shared__ float t_dz[MF_B], t_rz[MF_B], t_dz2[MF_B];
     __shared__ float t_L[MF_A * MF_B], t_y[MF_A * MF_B];
     __shared__ int red[2][4][4];
 
@@ -158,12 +158,14 @@ __global__ __launch_bounds__(256, FL_MATCHED_WG_PER_CU) void k_adj_matched_cone(
                 const float yd = f.ydet[A0 + k];
                 const float dx = (c * f.OD - s * yd) - x1, dy = (s * f.OD + c * yd) - y1;
                 t_dx[k] = dx; t_dy[k] = dy;
+                t_dxy2[k] = dx * dx + dy * dy;   // the first two terms of |d|^2, summed as the forward sums them
                 t_rx[k] = 1.0f / dx; t_ry[k] = 1.0f / dy;
             }
             if (tid >= 64 && tid - 64 < nB) {
                 const int k = tid - 64;
                 const float dz = (f.zdet[B0 + k] + sz) - z1;
                 t_dz[k] = dz;
+                t_dz2[k] = dz * dz;
                 t_rz[k] = 1.0f / dz;
             }
             __syncthreads();
@@ -188,8 +190,9 @@ __global__ __launch_bounds__(256, FL_MATCHED_WG_PER_CU) void k_adj_matched_cone(
             const uint32_t mA = fl_small_div_m((uint32_t)nA);   // k * nA < 2^18 in these tables
             for (int k = tid; k < nA * nB; k += 256) {
                 const int kb = fl_small_div(k, mA), ka = k - kb * nA;
-                const float dx = t_dx[ka], dy = t_dy[ka], dz = t_dz[kb];
-                t_L[k] = sqrtf(dx * dx + dy * dy + dz * dz);
+                // |d| = sqrtf(dx*dx + dy*dy + dz*dz), the forward's ray length
+                // bit for bit (left-to-right sum, correctly rounded root)
+                t_L[k] = sqrtf(t_dxy2[ka] + t_dz2[kb]);
                 t_y[k] = fl_load(y, (size_t)(B0 + kb) * na + (A0 + ka));
             }
             __syncthreads();
