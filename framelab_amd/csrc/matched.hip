@@ -105,6 +105,17 @@ __global__ __launch_bounds__(256, FL_MATCHED_WG_PER_CU) void k_adj_matched_cone(
 #pragma unroll
     for (int k = 0; k < KZ; k++) acc[k] = 0.0f;
     const int nvf = g.Nv[it];
+    // The shadow of a thread's (x, y) column is the same in all four waves
+    // (lane -> column is wave-independent): one wave per view computes it a
+    // view ahead, in turn, into double-buffered LDS, written before and read
+    // after the view's first barrier.
+    __shared__ __align__(16) ShadowCols s_cols[2][64];
+    auto col_shadow = [&](int jj) {
+        const int ivv = g.id_Y[it * g.tmp_size + jj];
+        return fl_shadow_cols(f.cs[ivv], f.sn[ivv], f.SO, SD, xc, yc, ma, na);
+    };
+    if (wv == 0 && nvf > 0) s_cols[0][lane] = col_shadow(0);
+    __syncthreads();
     for (int j = 0; j < nvf; j++) {
         const int iv = g.id_Y[it * g.tmp_size + j];
         const YT* __restrict__ y = Y + (size_t)g.id_S[it * g.tmp_size + j] * g.nd;
@@ -114,8 +125,9 @@ __global__ __launch_bounds__(256, FL_MATCHED_WG_PER_CU) void k_adj_matched_cone(
         int blo[KZ], bhi[KZ];
 #pragma unroll
         for (int k = 0; k < KZ; k++) { blo[k] = nb; bhi[k] = -1; }
+        const ShadowCols sc = s_cols[j & 1][lane];
+        if (wv == (j & 3) && j + 1 < nvf) s_cols[(j + 1) & 1][lane] = col_shadow(j + 1);
         if (col) {
-            const ShadowCols sc = fl_shadow_cols(c, s, f.SO, SD, xc, yc, ma, na);
             alo = sc.alo;
             ahi = sc.ahi;
 #pragma unroll
