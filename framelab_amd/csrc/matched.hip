@@ -67,8 +67,9 @@ __device__ __forceinline__ int mwave_max(int v) { return __ockl_wfred_max_i32(v)
 // footprint row the entry and exit parameters of each of its 4*KZ slices'
 // z planes (each the forward's IEEE division, done once per workgroup),
 // the ray lengths |d| and the sinogram values; each
-// voxel then visits only the cells of its own shadow.  Views of each frame
-// in order, columns outer; FP32 accumulation.
+// voxel then visits only the cells of its own shadow (the shadow of its
+// column is shared by the four waves through LDS).  Views of each frame in
+// order, columns outer; FP32 accumulation, one fused multiply-add per term.
 // Five workgroups per CU (measured optimum of the register budget).
 template <typename YT, typename XT, int KZ, bool DZ1>
 __global__ __launch_bounds__(256, FL_MATCHED_WG_PER_CU) void k_adj_matched_cone(const YT* __restrict__ Y, XT* __restrict__ X,
