@@ -546,20 +546,24 @@ def test_zero_volume_helix(gpu):
     assert int(torch.count_nonzero(P.adjoint_f64(torch.zeros(P.sino_shape, dtype=torch.float64, device=gpu)))) == 0
 
 
+@pytest.mark.parametrize("dt", ["float32", "bfloat16", "float16"])
 @pytest.mark.parametrize("tile", range(8))
-def test_forward_launch_shapes(gpu, oracle, tile):
+def test_forward_launch_shapes(gpu, oracle, tile, dt):
     """Every cone-forward launch shape (ray tiles, wave shapes, and the
     plain, 8x8x1, 4x4x1, 4x4x4 and 4x4x2 brick layouts of the gathered
     volume) gives the oracle's forward bit for bit, on sides that are not
-    multiples of the bricks (10 x 14 x 6), with three image frames."""
+    multiples of the bricks (10 x 14 x 6), with three image frames; with
+    BF16 or FP16 volume storage (2-byte bricks) and an FP32 sinogram it
+    equals the oracle's forward of the same values widened to FP32."""
     from framelab_amd import geometry as G
-    _say(f"case: cone forward launch shape {tile}")
+    _say(f"case: cone forward launch shape {tile}, {dt} volume")
     p = G.with_frames(G.cone_para(40, 24, 9, N=(10, 14, 6), L=(2.0, 2.8, 1.2), SO=4.0, OD=4.0,
                                   Ly=G.full_fan_width(), Lz=G.full_cone_height()), np.arange(9) % 3)
     P = _proj(p)
     P.set_tile(tile)
     rng = np.random.default_rng(40 + tile)
-    X = rng.random(P.vol_shape).astype(np.float32)
-    y = P.forward(torch.from_numpy(X).to(gpu)).cpu().numpy().ravel()
+    xt = torch.from_numpy(rng.random(P.vol_shape).astype(np.float32)).to(getattr(torch, dt))
+    X = xt.float().numpy()
+    y = P.forward(xt.to(gpu), dtype=torch.float32).cpu().numpy().ravel()
     yo = oracle.siddon_cone_f32(X, p)
     assert bits_equal(y, yo), first_mismatch(y, yo)
