@@ -85,9 +85,9 @@ struct Bricks {
     // The offset (x/BX)*BV + x%BX + (y/BY)*row + (y%BY)*BX + (z/BZ)*plane +
     // (z%BZ)*BX*BY (BV = BX*BY*BZ) is formed as
     //   x + BX*y + BX*BY*z + (x/BX)*(BV-BX) + (y/BY)*(row-BX*BY) + (z/BZ)*(plane-BV)
-    // (x%BX = x - (x/BX)*BX, likewise y and z): two shift-adds, three shifts
-    // and three 24-bit multiply-adds.  Every partial sum is taken modulo 2^32
-    // and the offset itself is below 2^32, so it comes out exact.
+    // (x%BX = x - (x/BX)*BX, likewise y and z): shifts, shift-adds and three
+    // 24-bit products, about ten instructions.  Every partial sum is taken
+    // modulo 2^32 and the offset itself is below 2^32, so it comes out exact.
     __device__ __forceinline__ unsigned at(int tx, int ty, int tz) const
     {
         const unsigned x = (unsigned)tx, y = (unsigned)ty, z = (unsigned)tz;
